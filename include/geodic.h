@@ -1,0 +1,150 @@
+/* geodic.h — C ABI of the MI355X DIC subset-correlation engine.
+ *
+ * Plain pointers and sizes only; no C++ or framework types cross this
+ * boundary and no call unwinds.  Every entry point returns a dic_status
+ * (0 = success); dic_last_error() gives the message of the last failure on
+ * the calling thread.
+ *
+ * The engine replaces the FP64 CPU hot path of geopyv-dev (Rust crate
+ * `geopyv-dev` 0.2.4).  Each declaration names the reference interface it
+ * stands in for (path:line in that repository).  The PyO3 module
+ * `_geopyv_dev` is what binds those interfaces today; the Python mirror in
+ * geopyv-dev_amd/geopyv_dev binds these symbols with ctypes instead.
+ */
+#ifndef GEODIC_H
+#define GEODIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: src/error.rs:5-40 one-to-one, then the two panics of
+ *      subset.rs (1143 Hessian, 520/595 W_delta) and device failures ---- */
+typedef enum {
+    DIC_OK = 0,
+    DIC_ERR_IMAGE_LOAD = 1,          /* Error::ImageLoad */
+    DIC_ERR_FILE_NOT_FOUND = 2,      /* Error::FileNotFound */
+    DIC_ERR_INVALID_INPUT = 3,       /* Error::InvalidInput */
+    DIC_ERR_MESH_GENERATION = 4,     /* Error::MeshGeneration */
+    DIC_ERR_IO = 5,                  /* Error::Io */
+    DIC_ERR_INVALID_MAGIC = 6,       /* Error::InvalidMagic */
+    DIC_ERR_UNSUPPORTED_VERSION = 7, /* Error::UnsupportedVersion */
+    DIC_ERR_NOT_SOLVED = 8,          /* Error::NotSolved */
+    DIC_ERR_CANCELLED = 9,           /* Error::Cancelled */
+    DIC_ERR_SINGULAR_HESSIAN = 10,   /* panic "ICGN Hessian singular", subset.rs:1143 */
+    DIC_ERR_SINGULAR_WARP = 11,      /* panic "W_delta singular", subset.rs:520,595 */
+    DIC_ERR_DEVICE = 20,             /* HIP runtime failure */
+    DIC_ERR_OUT_OF_MEMORY = 21,      /* device allocation or HBM budget exceeded */
+    DIC_ERR_BUSY = 22                /* serving queue full */
+} dic_status;
+
+/* Pixel element types for image upload. */
+enum { DIC_PIX_U8 = 0, DIC_PIX_F64 = 1 };
+/* Where a buffer lives. */
+enum { DIC_MEM_HOST = 0, DIC_MEM_DEVICE = 1 };
+/* Local mask shapes: masks.rs:21-31 MaskShape. */
+enum { DIC_SHAPE_CIRCLE = 0, DIC_SHAPE_SQUARE = 1, DIC_SHAPE_SEMICIRCLE = 2 };
+/* Per-node kernel: mesh.rs:285 SolveMethod. */
+enum { DIC_METHOD_ICGN = 0, DIC_METHOD_FAGN = 1 };
+/* Engine precision.  PLAIN_F64 is the drop-in (FP64 throughout, the
+ * reference's pass structure); COMP_F32 is the batched engine (FP32
+ * interpolation from an LDS-staged FP32 coefficient window, FP64
+ * accumulation, FP64 Cholesky update). */
+enum { DIC_PREC_PLAIN_F64 = 0, DIC_PREC_COMP_F32 = 1 };
+
+typedef struct dic_engine dic_engine;
+typedef struct dic_image dic_image;
+typedef struct dic_batch dic_batch;
+
+/* Solve settings: the arguments of Subset::solve_icgn / solve_fagn
+ * (subset.rs:992-1022), plus the engine choice. */
+typedef struct {
+    int32_t method;          /* DIC_METHOD_* */
+    int32_t precision;       /* DIC_PREC_* */
+    int32_t max_iterations;  /* `max_iterations` */
+    int32_t want_history;    /* record (iteration, norm, zncc, znssd) per iteration */
+    double tolerance;        /* `tolerance`: quality_ok needs c_zncc >= tolerance */
+    double max_norm;         /* `max_norm`: converged when ||dp|| < max_norm */
+} dic_solve_params;
+
+/* One subset's outcome: SolveResult (subset.rs:146-191) plus the reference
+ * quantities Subset::new stores (subset.rs:885-899). */
+typedef struct {
+    double p[12];            /* warp, p_len = 2/6/12 entries used */
+    double c_zncc, c_znssd;
+    double norm;             /* last convergence norm (NaN if no iteration) */
+    double eta_u, eta_v;     /* eta_omitted (subset.rs:748-820) */
+    double f_m, delta_f, sssig, sigma;
+    int32_t iterations, converged, quality_ok, status;
+    int32_t n_px, order;
+} dic_subset_result;
+
+/* ---- engine ---- */
+/* Creates the engine on HIP device `device` with its own stream. */
+int dic_engine_create(int device, dic_engine** out);
+void dic_engine_destroy(dic_engine* e);
+int dic_engine_sync(dic_engine* e);
+/* Kernel time of the last prefilter and the last batch solve, measured with
+ * HIP events on the engine's stream (milliseconds). */
+int dic_engine_timing(dic_engine* e, double* prefilter_ms, double* solve_ms);
+/* Thread-local message of the last failure ("" if none). */
+const char* dic_last_error(void);
+/* Device bytes currently held by the engine's images and batches. */
+int64_t dic_engine_bytes_in_use(dic_engine* e);
+
+/* ---- images: Image::from_array (image.rs:83-86) -> get_c (image.rs:252-327) ----
+ * `pixels` is (h, w) row-major, x = column, y = row, values 0-255.  The
+ * coefficient array C is (h+2b, w+2b) FP64 on the device, with an FP32 copy
+ * for the batched engine.  border must be >= 3 (image.rs:52). */
+int dic_image_create(dic_engine* e, const void* pixels, int pix_type, int mem, int64_t h, int64_t w,
+                     int border, dic_image** out);
+/* New frame of the same shape into an existing image (Sequence::solve's
+ * per-frame Image::from_file, sequence.rs:914): reuses its device buffers. */
+int dic_image_update(dic_image* img, const void* pixels, int pix_type, int mem);
+/* Copies C back to the host: (h+2b)*(w+2b) doubles (Image.coef, image.rs:51). */
+int dic_image_coef(const dic_image* img, double* out);
+int dic_image_info(const dic_image* img, int64_t* h, int64_t* w, int* border);
+void dic_image_destroy(dic_image* img);
+
+/* ---- masks: LocalMask::new (masks.rs:58-187) and mask_update
+ *      (masks.rs:199-263), host-side and bit-exact ----
+ * coords_out holds (2*size+1)^2 pairs [x_off, y_off]; subset_mask_out
+ * (2*size+1)^2 int32 (either may be NULL). */
+int dic_mask_local(int shape, int size, double* coords_out, int32_t* subset_mask_out, int64_t* n_out);
+int dic_mask_update(int shape, int size, double cx, double cy, const uint8_t* mask, int64_t mh, int64_t mw,
+                    double* coords_out, int64_t* n_out);
+
+/* ---- batched subset solve: Subset::new (subset.rs:836-919) followed by
+ *      solve_icgn_result (subset.rs:1117-1219) or solve_fagn_result
+ *      (subset.rs:1228-1334), for many subsets in one launch ----
+ * Pixel lists are shared templates: template t holds the integer offsets
+ * tpl_xy[2*tpl_start[t] .. 2*tpl_start[t+1]) as (dx, dy) pairs, in the
+ * reference's row-major scan order; subset s uses template tpl_index[s]
+ * about centre coords[2s], coords[2s+1] with warp order orders[s]. */
+int dic_batch_create(dic_engine* e, int64_t n_subsets, const double* coords, const int32_t* orders,
+                     const int32_t* tpl_index, int32_t n_templates, const int64_t* tpl_start,
+                     const int16_t* tpl_xy, dic_batch** out);
+/* Initial warps, n_subsets x 12 doubles (missing entries zero; p0 is
+ * truncated or zero-padded to p_len as subset.rs:1135-1136 does). */
+int dic_batch_set_p0(dic_batch* b, const double* p0, int mem);
+/* Queues the solve on the engine stream; returns without waiting. */
+int dic_batch_solve(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
+/* Waits, then copies n_subsets results to the host. */
+int dic_batch_results(dic_batch* b, dic_subset_result* out);
+/* Waits, then copies n_subsets*max_iterations*4 history doubles
+ * (iteration, norm, c_zncc, c_znssd); rows past `iterations` are zero. */
+int dic_batch_history(dic_batch* b, double* out);
+/* Reference intensities f (n_px) and gradients grad_f (n_px x 2) of one
+ * subset (subset.rs:877-884); available after a DIC_PREC_PLAIN_F64 solve. */
+int dic_batch_subset_arrays(dic_batch* b, int64_t subset, double* f, double* grad_f);
+/* Final warps as a device pointer (n_subsets x 12 doubles) for chaining
+ * batches without a host round trip. */
+int dic_batch_device_p(dic_batch* b, const double** out);
+void dic_batch_destroy(dic_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEODIC_H */
