@@ -1,0 +1,485 @@
+// C ABI of the engine (include/geodic.h): engine, images, batches.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <string>
+
+#include "internal.h"
+
+namespace geodic {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int dev_alloc(Engine* e, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t st = hipMalloc(p, bytes);
+    if (st != hipSuccess) {
+        *p = nullptr;
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(st));
+        return st == hipErrorOutOfMemory ? DIC_ERR_OUT_OF_MEMORY : DIC_ERR_DEVICE;
+    }
+    e->bytes_in_use += (int64_t)bytes;
+    return DIC_OK;
+}
+
+void dev_free(Engine* e, void* p, size_t bytes) {
+    if (!p) return;
+    if (bytes == 0) bytes = 16;
+    (void)hipFree(p);
+    e->bytes_in_use -= (int64_t)bytes;
+}
+
+}  // namespace geodic
+
+using namespace geodic;
+
+struct dic_engine {
+    Engine e;
+};
+struct dic_image {
+    Image im;
+};
+struct dic_batch {
+    Batch b;
+};
+
+#define CHECK_ARG(cond, msg)                  \
+    do {                                      \
+        if (!(cond)) {                        \
+            set_error(std::string(msg));      \
+            return DIC_ERR_INVALID_INPUT;     \
+        }                                     \
+    } while (0)
+
+extern "C" {
+
+const char* dic_last_error(void) { return g_last_error.c_str(); }
+
+int dic_engine_create(int device, dic_engine** out) {
+    CHECK_ARG(out, "dic_engine_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    DIC_HIP_TRY(hipGetDeviceCount(&count));
+    CHECK_ARG(device >= 0 && device < count, "dic_engine_create: no such HIP device");
+    DIC_HIP_TRY(hipSetDevice(device));
+    dic_engine* d = new dic_engine();
+    Engine* e = &d->e;
+    e->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        e->num_cus = prop.multiProcessorCount;
+    // default priority stream (mixed stream priorities are avoided)
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->ev_pre0) != hipSuccess || hipEventCreate(&e->ev_pre1) != hipSuccess ||
+        hipEventCreate(&e->ev_sol0) != hipSuccess || hipEventCreate(&e->ev_sol1) != hipSuccess) {
+        set_error("dic_engine_create: stream/event creation failed");
+        delete d;
+        return DIC_ERR_DEVICE;
+    }
+    *out = d;
+    return DIC_OK;
+}
+
+void dic_engine_destroy(dic_engine* d) {
+    if (!d) return;
+    Engine* e = &d->e;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    for (auto& pr : e->twiddles) (void)hipFree(pr.second);
+    if (e->pre_tmp) (void)hipFree(e->pre_tmp);
+    if (e->pix_stage) (void)hipFree(e->pix_stage);
+    (void)hipEventDestroy(e->ev_pre0);
+    (void)hipEventDestroy(e->ev_pre1);
+    (void)hipEventDestroy(e->ev_sol0);
+    (void)hipEventDestroy(e->ev_sol1);
+    (void)hipStreamDestroy(e->stream);
+    delete d;
+}
+
+int dic_engine_sync(dic_engine* d) {
+    CHECK_ARG(d, "engine is NULL");
+    DIC_HIP_TRY(hipSetDevice(d->e.device));
+    DIC_HIP_TRY(hipStreamSynchronize(d->e.stream));
+    return DIC_OK;
+}
+
+int dic_engine_timing(dic_engine* d, double* prefilter_ms, double* solve_ms) {
+    CHECK_ARG(d, "engine is NULL");
+    Engine* e = &d->e;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    float ms = 0.f;
+    if (prefilter_ms) {
+        *prefilter_ms = 0.0;
+        if (e->pre_recorded) {
+            DIC_HIP_TRY(hipEventElapsedTime(&ms, e->ev_pre0, e->ev_pre1));
+            *prefilter_ms = ms;
+        }
+    }
+    if (solve_ms) {
+        *solve_ms = 0.0;
+        if (e->sol_recorded) {
+            DIC_HIP_TRY(hipEventElapsedTime(&ms, e->ev_sol0, e->ev_sol1));
+            *solve_ms = ms;
+        }
+    }
+    return DIC_OK;
+}
+
+int64_t dic_engine_bytes_in_use(dic_engine* d) { return d ? d->e.bytes_in_use : 0; }
+
+// ---------------------------------------------------------------------------
+// images
+// ---------------------------------------------------------------------------
+int dic_image_create(dic_engine* d, const void* pixels, int pix_type, int mem, int64_t h, int64_t w, int border,
+                     dic_image** out) {
+    CHECK_ARG(d && out && pixels, "dic_image_create: NULL argument");
+    *out = nullptr;
+    CHECK_ARG(h >= 1 && w >= 1, "invalid input: image must be at least 1x1");
+    CHECK_ARG(border >= 3, "invalid input: border must be >= 3");
+    CHECK_ARG(pix_type == DIC_PIX_U8 || pix_type == DIC_PIX_F64, "invalid input: pixel type");
+    CHECK_ARG(mem == DIC_MEM_HOST || mem == DIC_MEM_DEVICE, "invalid input: memory kind");
+    Engine* e = &d->e;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    dic_image* di = new dic_image();
+    Image* im = &di->im;
+    im->eng = e;
+    im->h = h;
+    im->w = w;
+    im->border = border;
+    im->hp = h + 2 * border;
+    im->wp = w + 2 * border;
+    const size_t np = (size_t)im->hp * im->wp;
+    int st = dev_alloc(e, (void**)&im->coef64, np * sizeof(double));
+    if (!st) st = dev_alloc(e, (void**)&im->coef32, np * sizeof(float));
+    if (!st) st = prefilter_run(e, im, pixels, pix_type, mem);
+    if (!st && mem == DIC_MEM_HOST && hipStreamSynchronize(e->stream) != hipSuccess) {
+        set_error("dic_image_create: synchronisation failed");
+        st = DIC_ERR_DEVICE;
+    }
+    if (st) {
+        dev_free(e, im->coef64, np * sizeof(double));
+        dev_free(e, im->coef32, np * sizeof(float));
+        delete di;
+        return st;
+    }
+    *out = di;
+    return DIC_OK;
+}
+
+int dic_image_update(dic_image* di, const void* pixels, int pix_type, int mem) {
+    CHECK_ARG(di && pixels, "dic_image_update: NULL argument");
+    CHECK_ARG(pix_type == DIC_PIX_U8 || pix_type == DIC_PIX_F64, "invalid input: pixel type");
+    CHECK_ARG(mem == DIC_MEM_HOST || mem == DIC_MEM_DEVICE, "invalid input: memory kind");
+    Image* im = &di->im;
+    Engine* e = im->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    int st = prefilter_run(e, im, pixels, pix_type, mem);
+    if (st) return st;
+    if (mem == DIC_MEM_HOST) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_image_coef(const dic_image* di, double* out) {
+    CHECK_ARG(di && out, "dic_image_coef: NULL argument");
+    const Image* im = &di->im;
+    Engine* e = im->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    DIC_HIP_TRY(hipMemcpyAsync(out, im->coef64, sizeof(double) * im->hp * im->wp, hipMemcpyDeviceToHost, e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_image_info(const dic_image* di, int64_t* h, int64_t* w, int* border) {
+    CHECK_ARG(di, "dic_image_info: NULL image");
+    if (h) *h = di->im.h;
+    if (w) *w = di->im.w;
+    if (border) *border = di->im.border;
+    return DIC_OK;
+}
+
+void dic_image_destroy(dic_image* di) {
+    if (!di) return;
+    Image* im = &di->im;
+    Engine* e = im->eng;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    const size_t np = (size_t)im->hp * im->wp;
+    dev_free(e, im->coef64, np * sizeof(double));
+    dev_free(e, im->coef32, np * sizeof(float));
+    delete di;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+static void batch_free(Batch* b) {
+    Engine* e = b->eng;
+    const int64_t n = b->n;
+    dev_free(e, b->d_coords, sizeof(double) * 2 * n);
+    dev_free(e, b->d_orders, sizeof(int32_t) * n);
+    dev_free(e, b->d_tpl_index, sizeof(int32_t) * n);
+    dev_free(e, b->d_tpl_start, sizeof(int64_t) * (b->n_tpl + 1));
+    dev_free(e, b->d_tpl_xy, sizeof(int32_t) * (b->h_tpl_start.empty() ? 0 : b->h_tpl_start.back()));
+    dev_free(e, b->d_tpl_ext, sizeof(int32_t) * 4 * b->n_tpl);
+    dev_free(e, b->d_scr_off, sizeof(int64_t) * n);
+    dev_free(e, b->d_p0, sizeof(double) * 12 * n);
+    dev_free(e, b->d_p, sizeof(double) * 12 * n);
+    dev_free(e, b->d_res, sizeof(dic_subset_result) * n);
+    dev_free(e, b->d_hist, sizeof(double) * 4 * n * b->hist_cap);
+    dev_free(e, b->d_scratch, sizeof(double) * b->scratch_doubles);
+    dev_free(e, b->d_slab, sizeof(float) * b->slab_floats);
+    dev_free(e, b->d_slab_off, sizeof(int64_t) * n);
+    dev_free(e, b->d_order_perm, sizeof(int32_t) * n);
+    dev_free(e, b->d_queue, sizeof(int));
+}
+
+int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, const int32_t* orders,
+                     const int32_t* tpl_index, int32_t n_templates, const int64_t* tpl_start, const int16_t* tpl_xy,
+                     dic_batch** out) {
+    CHECK_ARG(d && out, "dic_batch_create: NULL argument");
+    *out = nullptr;
+    CHECK_ARG(n_subsets >= 1, "invalid input: batch needs at least one subset");
+    CHECK_ARG(n_subsets <= (int64_t)1 << 30, "invalid input: batch too large");
+    CHECK_ARG(coords && orders && tpl_index && tpl_start && tpl_xy, "dic_batch_create: NULL array");
+    CHECK_ARG(n_templates >= 1, "invalid input: no templates");
+    CHECK_ARG(tpl_start[0] == 0, "invalid input: tpl_start[0] must be 0");
+    for (int t = 0; t < n_templates; ++t)
+        CHECK_ARG(tpl_start[t + 1] >= tpl_start[t], "invalid input: tpl_start must be non-decreasing");
+    const int order0 = orders[0];
+    for (int64_t s = 0; s < n_subsets; ++s) {
+        CHECK_ARG(orders[s] >= 0 && orders[s] <= 2, "invalid input: subset_order must be 0, 1, or 2");
+        CHECK_ARG(orders[s] == order0, "invalid input: all subsets of one batch share the warp order");
+        CHECK_ARG(tpl_index[s] >= 0 && tpl_index[s] < n_templates, "invalid input: template index out of range");
+    }
+    Engine* e = &d->e;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    dic_batch* db = new dic_batch();
+    Batch* b = &db->b;
+    b->eng = e;
+    b->n = n_subsets;
+    b->n_tpl = n_templates;
+    b->max_order = order0;
+    b->h_tpl_start.assign(tpl_start, tpl_start + n_templates + 1);
+    b->h_tpl_index.assign(tpl_index, tpl_index + n_subsets);
+    const int64_t n_off = tpl_start[n_templates];
+    std::vector<int32_t> packed(n_off);
+    std::vector<int32_t> ext(4 * (size_t)n_templates, 0);
+    for (int t = 0; t < n_templates; ++t) {
+        int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+        for (int64_t i = tpl_start[t]; i < tpl_start[t + 1]; ++i) {
+            const int dx = tpl_xy[2 * i], dy = tpl_xy[2 * i + 1];
+            if (i == tpl_start[t]) {
+                x0 = x1 = dx;
+                y0 = y1 = dy;
+            } else {
+                x0 = std::min(x0, dx);
+                x1 = std::max(x1, dx);
+                y0 = std::min(y0, dy);
+                y1 = std::max(y1, dy);
+            }
+            packed[i] = (int32_t)(((uint32_t)(uint16_t)dx) | ((uint32_t)(uint16_t)dy << 16));
+        }
+        ext[4 * t] = x0;
+        ext[4 * t + 1] = x1;
+        ext[4 * t + 2] = y0;
+        ext[4 * t + 3] = y1;
+        const int64_t nt = tpl_start[t + 1] - tpl_start[t];
+        b->max_n = std::max<int32_t>(b->max_n, (int32_t)nt);
+        b->max_side = std::max(b->max_side, std::max(x1 - x0 + 1, y1 - y0 + 1));
+    }
+    for (int64_t s = 0; s < n_subsets; ++s) {
+        const int t = tpl_index[s];
+        b->total_px += tpl_start[t + 1] - tpl_start[t];
+    }
+    // largest subsets first: the persistent grid drains the tail evenly
+    std::vector<int32_t> perm(n_subsets);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t c) {
+        const int64_t na = tpl_start[tpl_index[a] + 1] - tpl_start[tpl_index[a]];
+        const int64_t nc = tpl_start[tpl_index[c] + 1] - tpl_start[tpl_index[c]];
+        return na > nc;
+    });
+    int st = DIC_OK;
+    auto A = [&](void** p, size_t bytes) {
+        if (!st) st = dev_alloc(e, p, bytes);
+    };
+    A((void**)&b->d_coords, sizeof(double) * 2 * n_subsets);
+    A((void**)&b->d_orders, sizeof(int32_t) * n_subsets);
+    A((void**)&b->d_tpl_index, sizeof(int32_t) * n_subsets);
+    A((void**)&b->d_tpl_start, sizeof(int64_t) * (n_templates + 1));
+    A((void**)&b->d_tpl_xy, sizeof(int32_t) * n_off);
+    A((void**)&b->d_tpl_ext, sizeof(int32_t) * 4 * n_templates);
+    A((void**)&b->d_scr_off, sizeof(int64_t) * n_subsets);
+    A((void**)&b->d_slab_off, sizeof(int64_t) * n_subsets);
+    A((void**)&b->d_p0, sizeof(double) * 12 * n_subsets);
+    A((void**)&b->d_p, sizeof(double) * 12 * n_subsets);
+    A((void**)&b->d_res, sizeof(dic_subset_result) * n_subsets);
+    A((void**)&b->d_order_perm, sizeof(int32_t) * n_subsets);
+    A((void**)&b->d_queue, sizeof(int));
+    auto H2D = [&](void* dst, const void* src, size_t bytes) {
+        if (!st && bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess) {
+            set_error("dic_batch_create: upload failed");
+            st = DIC_ERR_DEVICE;
+        }
+    };
+    H2D(b->d_coords, coords, sizeof(double) * 2 * n_subsets);
+    H2D(b->d_orders, orders, sizeof(int32_t) * n_subsets);
+    H2D(b->d_tpl_index, tpl_index, sizeof(int32_t) * n_subsets);
+    H2D(b->d_tpl_start, tpl_start, sizeof(int64_t) * (n_templates + 1));
+    H2D(b->d_tpl_xy, packed.data(), sizeof(int32_t) * n_off);
+    H2D(b->d_tpl_ext, ext.data(), sizeof(int32_t) * 4 * n_templates);
+    H2D(b->d_order_perm, perm.data(), sizeof(int32_t) * n_subsets);
+    if (!st && hipMemsetAsync(b->d_p0, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
+    if (!st && hipMemsetAsync(b->d_p, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
+    if (!st && hipMemsetAsync(b->d_res, 0, sizeof(dic_subset_result) * n_subsets, e->stream) != hipSuccess)
+        st = DIC_ERR_DEVICE;
+    // host vectors must stay alive until the copies land
+    if (!st && hipStreamSynchronize(e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
+    if (st) {
+        batch_free(b);
+        delete db;
+        return st;
+    }
+    *out = db;
+    return DIC_OK;
+}
+
+int dic_batch_set_p0(dic_batch* db, const double* p0, int mem) {
+    CHECK_ARG(db && p0, "dic_batch_set_p0: NULL argument");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    DIC_HIP_TRY(hipMemcpyAsync(b->d_p0, p0, sizeof(double) * 12 * b->n,
+                               mem == DIC_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, e->stream));
+    if (mem == DIC_MEM_HOST) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_cu) {
+    CHECK_ARG(db, "dic_batch_set_schedule: NULL batch");
+    CHECK_ARG(win_slack >= 0 && win_slack <= 64, "invalid input: win_slack must be in 0..64");
+    CHECK_ARG(waves_per_cu >= 1 && waves_per_cu <= 32, "invalid input: waves_per_cu must be in 1..32");
+    db->b.win_slack = win_slack;
+    db->b.waves_per_cu = waves_per_cu;
+    return DIC_OK;
+}
+
+int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const dic_solve_params* prm) {
+    CHECK_ARG(db && f && g && prm, "dic_batch_solve: NULL argument");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    CHECK_ARG(f->im.eng == e && g->im.eng == e, "invalid input: images belong to another engine");
+    CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
+              "invalid input: reference and target images differ in shape or border");
+    CHECK_ARG(prm->method == DIC_METHOD_ICGN || prm->method == DIC_METHOD_FAGN, "invalid input: method");
+    CHECK_ARG(prm->precision == DIC_PREC_PLAIN_F64 || prm->precision == DIC_PREC_COMP_F32, "invalid input: precision");
+    CHECK_ARG(prm->max_iterations >= 0 && prm->max_iterations <= 100000, "invalid input: max_iterations");
+    CHECK_ARG(!(prm->precision == DIC_PREC_COMP_F32 && prm->method == DIC_METHOD_FAGN),
+              "invalid input: the batched engine runs ICGN; use DIC_PREC_PLAIN_F64 for FAGN");
+    CHECK_ARG(b->max_n <= 32767 * 2, "invalid input: template too large");
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (prm->want_history) {
+        const int cap = std::max(prm->max_iterations, 1);
+        if (b->hist_cap != cap) {
+            dev_free(e, b->d_hist, sizeof(double) * 4 * b->n * b->hist_cap);
+            b->d_hist = nullptr;
+            b->hist_cap = 0;
+            int st = dev_alloc(e, (void**)&b->d_hist, sizeof(double) * 4 * b->n * cap);
+            if (st) return st;
+            b->hist_cap = cap;
+        }
+        DIC_HIP_TRY(hipMemsetAsync(b->d_hist, 0, sizeof(double) * 4 * b->n * b->hist_cap, e->stream));
+    }
+    int st;
+    if (prm->precision == DIC_PREC_PLAIN_F64) {
+        // scratch sizing and offsets are host work done before the timed launch
+        DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
+        st = launch_icgn_plain(b, &f->im, &g->im, prm);
+    } else {
+        DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
+        st = launch_icgn_batch(b, &f->im, &g->im, prm);
+    }
+    if (st) return st;
+    DIC_HIP_TRY(hipEventRecord(e->ev_sol1, e->stream));
+    e->sol_recorded = true;
+    b->last_precision = prm->precision;
+    b->last_method = prm->method;
+    b->last_max_it = prm->max_iterations;
+    b->last_history = prm->want_history != 0;
+    return DIC_OK;
+}
+
+int dic_batch_results(dic_batch* db, dic_subset_result* out) {
+    CHECK_ARG(db && out, "dic_batch_results: NULL argument");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    DIC_HIP_TRY(hipMemcpyAsync(out, b->d_res, sizeof(dic_subset_result) * b->n, hipMemcpyDeviceToHost, e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_batch_history(dic_batch* db, double* out) {
+    CHECK_ARG(db && out, "dic_batch_history: NULL argument");
+    Batch* b = &db->b;
+    if (!b->last_history || !b->d_hist) {
+        set_error("not solved: no history was recorded (want_history = 0)");
+        return DIC_ERR_NOT_SOLVED;
+    }
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    DIC_HIP_TRY(hipMemcpyAsync(out, b->d_hist, sizeof(double) * 4 * b->n * b->hist_cap, hipMemcpyDeviceToHost,
+                               e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_batch_subset_arrays(dic_batch* db, int64_t s, double* f, double* grad_f) {
+    CHECK_ARG(db, "dic_batch_subset_arrays: NULL batch");
+    Batch* b = &db->b;
+    CHECK_ARG(s >= 0 && s < b->n, "invalid input: subset index out of range");
+    if (b->last_precision != DIC_PREC_PLAIN_F64 || b->h_scr_off.empty()) {
+        set_error("not solved: per-pixel arrays need a DIC_PREC_PLAIN_F64 solve");
+        return DIC_ERR_NOT_SOLVED;
+    }
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    const int t = b->h_tpl_index[s];
+    const int64_t n = b->h_tpl_start[t + 1] - b->h_tpl_start[t];
+    std::vector<double> buf(3 * n);
+    DIC_HIP_TRY(hipMemcpyAsync(buf.data(), b->d_scratch + b->h_scr_off[s], sizeof(double) * 3 * n,
+                               hipMemcpyDeviceToHost, e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (f) std::memcpy(f, buf.data(), sizeof(double) * n);
+    if (grad_f)
+        for (int64_t i = 0; i < n; ++i) {
+            grad_f[2 * i] = buf[n + i];
+            grad_f[2 * i + 1] = buf[2 * n + i];
+        }
+    return DIC_OK;
+}
+
+int dic_batch_device_p(dic_batch* db, const double** out) {
+    CHECK_ARG(db && out, "dic_batch_device_p: NULL argument");
+    *out = db->b.d_p;
+    return DIC_OK;
+}
+
+void dic_batch_destroy(dic_batch* db) {
+    if (!db) return;
+    Batch* b = &db->b;
+    (void)hipSetDevice(b->eng->device);
+    (void)hipStreamSynchronize(b->eng->stream);
+    batch_free(b);
+    delete db;
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// Bi-quintic B-spline basis and coefficient-block evaluation, device side.
+//
+// C is the (h+2b) x (w+2b) coefficient array; the 6x6 block of pixel
+// (yf, xf) starts at C[yf+b-2, xf+b-2] (image.rs:48-51).  A query (x, y)
+// is clamped to [0, w-1e-9] x [0, h-1e-9] before the floor (image.rs:
+// 130-131), so out-of-range queries saturate to the boundary value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace geodic {
+
+// Q (image.rs:183-190): row r = power of t, column k = tap.
+#define GQ00 (1.0 / 120.0)
+#define GQ01 (13.0 / 60.0)
+#define GQ02 (11.0 / 20.0)
+#define GQ03 (13.0 / 60.0)
+#define GQ04 (1.0 / 120.0)
+#define GQ10 (-1.0 / 24.0)
+#define GQ11 (-5.0 / 12.0)
+#define GQ13 (5.0 / 12.0)
+#define GQ14 (1.0 / 24.0)
+#define GQ20 (1.0 / 12.0)
+#define GQ21 (1.0 / 6.0)
+#define GQ22 (-1.0 / 2.0)
+#define GQ23 (1.0 / 6.0)
+#define GQ24 (1.0 / 12.0)
+#define GQ30 (-1.0 / 12.0)
+#define GQ31 (1.0 / 6.0)
+#define GQ33 (-1.0 / 6.0)
+#define GQ34 (1.0 / 12.0)
+#define GQ40 (1.0 / 24.0)
+#define GQ41 (-1.0 / 6.0)
+#define GQ42 (1.0 / 4.0)
+#define GQ43 (-1.0 / 6.0)
+#define GQ44 (1.0 / 24.0)
+#define GQ50 (-1.0 / 120.0)
+#define GQ51 (1.0 / 24.0)
+#define GQ52 (-1.0 / 12.0)
+#define GQ53 (1.0 / 12.0)
+#define GQ54 (-1.0 / 24.0)
+#define GQ55 (1.0 / 120.0)
+
+__device__ __forceinline__ double clamp_ref(double v, double lo, double hi) {
+    // f64::clamp: NaN passes through
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Qᵀ·[1, t, ..., t⁵] with the reference's power-sum association
+// (image.rs:194-208): out[k] = Σ_r Q[r][k]·t^r, r ascending, from 0.0.
+__device__ __forceinline__ void qt_poly_ref(double t, double o[6]) {
+#pragma clang fp contract(off)
+    const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+    o[0] = ((((0.0 + GQ00) + GQ10 * t) + GQ20 * t2) + GQ30 * t3 + GQ40 * t4) + GQ50 * t5;
+    o[1] = ((((0.0 + GQ01) + GQ11 * t) + GQ21 * t2) + GQ31 * t3 + GQ41 * t4) + GQ51 * t5;
+    o[2] = ((((0.0 + GQ02) + 0.0 * t) + GQ22 * t2) + 0.0 * t3 + GQ42 * t4) + GQ52 * t5;
+    o[3] = ((((0.0 + GQ03) + GQ13 * t) + GQ23 * t2) + GQ33 * t3 + GQ43 * t4) + GQ53 * t5;
+    o[4] = ((((0.0 + GQ04) + GQ14 * t) + GQ24 * t2) + GQ34 * t3 + GQ44 * t4) + GQ54 * t5;
+    o[5] = GQ55 * t5;
+}
+
+// Qᵀ·[0, 1, 2t, 3t², 4t³, 5t⁴] (image.rs:217-230)
+__device__ __forceinline__ void qt_dpoly_ref(double t, double o[6]) {
+#pragma clang fp contract(off)
+    const double t2 = t * t, t3 = t2 * t;
+    const double p1 = 1.0, p2 = 2.0 * t, p3 = 3.0 * t2, p4 = 4.0 * t3, p5 = 5.0 * t2 * t2;
+    o[0] = (((GQ10 * p1 + GQ20 * p2) + GQ30 * p3) + GQ40 * p4) + GQ50 * p5;
+    o[1] = (((GQ11 * p1 + GQ21 * p2) + GQ31 * p3) + GQ41 * p4) + GQ51 * p5;
+    o[2] = (((0.0 * p1 + GQ22 * p2) + 0.0 * p3) + GQ42 * p4) + GQ52 * p5;
+    o[3] = (((GQ13 * p1 + GQ23 * p2) + GQ33 * p3) + GQ43 * p4) + GQ53 * p5;
+    o[4] = (((GQ14 * p1 + GQ24 * p2) + GQ34 * p3) + GQ44 * p4) + GQ54 * p5;
+    o[5] = GQ55 * p5;
+}
+
+// FP64 value from the global coefficient array (image.rs:128-148).
+__device__ __forceinline__ double eval_f64(const double* __restrict__ C, int64_t wp, int border, double wmax,
+                                           double hmax, double x, double y) {
+#pragma clang fp contract(off)
+    x = clamp_ref(x, 0.0, wmax);
+    y = clamp_ref(y, 0.0, hmax);
+    // after the clamp only NaN fails x >= 0: it indexes cell 0 with a NaN
+    // fraction, as the reference's saturating `as usize` cast does
+    const double fx = (x >= 0.0) ? floor(x) : 0.0, fy = (y >= 0.0) ? floor(y) : 0.0;
+    double a[6], b[6];
+    qt_poly_ref(y - fy, a);
+    qt_poly_ref(x - fx, b);
+    const double* base = C + ((int64_t)fy + border - 2) * wp + ((int64_t)fx + border - 2);
+    double val = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        const double* row = base + r * wp;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s += row[k] * b[k];
+        val += a[r] * s;
+    }
+    return val;
+}
+
+// FP64 value and gradient (image.rs:115-124,155-178).
+__device__ __forceinline__ void eval_grad_f64(const double* __restrict__ C, int64_t wp, int border, double wmax,
+                                              double hmax, double x, double y, double& v, double& gx, double& gy) {
+#pragma clang fp contract(off)
+    x = clamp_ref(x, 0.0, wmax);
+    y = clamp_ref(y, 0.0, hmax);
+    // after the clamp only NaN fails x >= 0: it indexes cell 0 with a NaN
+    // fraction, as the reference's saturating `as usize` cast does
+    const double fx = (x >= 0.0) ? floor(x) : 0.0, fy = (y >= 0.0) ? floor(y) : 0.0;
+    double a[6], da[6], b[6], db[6];
+    qt_poly_ref(y - fy, a);
+    qt_dpoly_ref(y - fy, da);
+    qt_poly_ref(x - fx, b);
+    qt_dpoly_ref(x - fx, db);
+    const double* base = C + ((int64_t)fy + border - 2) * wp + ((int64_t)fx + border - 2);
+    double val = 0.0, ggx = 0.0, ggy = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        const double* row = base + r * wp;
+        double s = 0.0, t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            s += row[k] * b[k];
+            t += row[k] * db[k];
+        }
+        val += a[r] * s;
+        ggx += a[r] * t;
+        ggy += da[r] * s;
+    }
+    v = val;
+    gx = ggx;
+    gy = ggy;
+}
+
+// FP32 basis in Horner form (same polynomials as qt_poly / qt_dpoly).
+__device__ __forceinline__ void basis_f32(float t, float b[6]) {
+    b[0] = (float)GQ00 + t * ((float)GQ10 + t * ((float)GQ20 + t * ((float)GQ30 + t * ((float)GQ40 + t * (float)GQ50))));
+    b[1] = (float)GQ01 + t * ((float)GQ11 + t * ((float)GQ21 + t * ((float)GQ31 + t * ((float)GQ41 + t * (float)GQ51))));
+    b[2] = (float)GQ02 + t * t * ((float)GQ22 + t * t * ((float)GQ42 + t * (float)GQ52));
+    b[3] = (float)GQ03 + t * ((float)GQ13 + t * ((float)GQ23 + t * ((float)GQ33 + t * ((float)GQ43 + t * (float)GQ53))));
+    b[4] = (float)GQ04 + t * ((float)GQ14 + t * ((float)GQ24 + t * ((float)GQ34 + t * ((float)GQ44 + t * (float)GQ54))));
+    const float t2 = t * t;
+    b[5] = (float)GQ55 * (t2 * t2 * t);
+}
+
+__device__ __forceinline__ void dbasis_f32(float t, float d[6]) {
+    d[0] = (float)GQ10 + t * (2.f * (float)GQ20 + t * (3.f * (float)GQ30 + t * (4.f * (float)GQ40 + t * 5.f * (float)GQ50)));
+    d[1] = (float)GQ11 + t * (2.f * (float)GQ21 + t * (3.f * (float)GQ31 + t * (4.f * (float)GQ41 + t * 5.f * (float)GQ51)));
+    d[2] = t * (2.f * (float)GQ22 + t * t * (4.f * (float)GQ42 + t * 5.f * (float)GQ52));
+    d[3] = (float)GQ13 + t * (2.f * (float)GQ23 + t * (3.f * (float)GQ33 + t * (4.f * (float)GQ43 + t * 5.f * (float)GQ53)));
+    d[4] = (float)GQ14 + t * (2.f * (float)GQ24 + t * (3.f * (float)GQ34 + t * (4.f * (float)GQ44 + t * 5.f * (float)GQ54)));
+    const float t2 = t * t;
+    d[5] = 5.f * (float)GQ55 * (t2 * t2);
+}
+
+// Wavefront (64-lane) butterfly sum; every lane receives the same total and
+// the tree order is fixed, so results do not depend on what else runs.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Value known to be equal on every lane, moved to scalar registers.
+__device__ __forceinline__ double uni(double v) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Butterfly sum returned in scalar registers.
+__device__ __forceinline__ double wave_sum_u(double v) { return uni(wave_sum(v)); }
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int unpack_dx(int32_t v) { return (int)(int16_t)(v & 0xffff); }
+__device__ __forceinline__ int unpack_dy(int32_t v) { return (int)(int16_t)((uint32_t)v >> 16); }
+
+template <int M>
+struct PLen {};
+
+}  // namespace geodic

@@ -1,0 +1,107 @@
+// Internal declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/geodic.h"
+
+namespace geodic {
+
+// Thread-local message for dic_last_error().
+void set_error(const std::string& msg);
+
+#define DIC_HIP_TRY(expr)                                                                   \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess) {                                                             \
+            ::geodic::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));         \
+            return (_e == hipErrorOutOfMemory) ? DIC_ERR_OUT_OF_MEMORY : DIC_ERR_DEVICE;    \
+        }                                                                                   \
+    } while (0)
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_pre0 = nullptr, ev_pre1 = nullptr, ev_sol0 = nullptr, ev_sol1 = nullptr;
+    bool pre_recorded = false, sol_recorded = false;
+    int64_t bytes_in_use = 0;
+    int num_cus = 256;
+    std::mutex mu;
+    // cached twiddle tables per FFT length (device, complex double)
+    std::vector<std::pair<int, double2*>> twiddles;
+    // prefilter intermediate (row-pass output), grown on demand
+    double* pre_tmp = nullptr;
+    size_t pre_tmp_bytes = 0;
+    // pixel staging for host uploads
+    void* pix_stage = nullptr;
+    size_t pix_stage_bytes = 0;
+};
+
+struct Image {
+    Engine* eng = nullptr;
+    int64_t h = 0, w = 0;
+    int border = 0;
+    int64_t hp = 0, wp = 0;  // padded shape
+    double* coef64 = nullptr;
+    float* coef32 = nullptr;
+};
+
+struct Batch {
+    Engine* eng = nullptr;
+    int64_t n = 0;
+    int32_t n_tpl = 0;
+    int64_t total_px = 0;     // Σ n_px over subsets (scratch sizing)
+    int32_t max_n = 0;        // largest template
+    int32_t max_order = 0;
+    // device arrays
+    double* d_coords = nullptr;     // n x 2
+    int32_t* d_orders = nullptr;    // n
+    int32_t* d_tpl_index = nullptr; // n
+    int64_t* d_tpl_start = nullptr; // n_tpl + 1
+    int32_t* d_tpl_xy = nullptr;    // packed (dx & 0xffff) | (dy << 16)
+    int32_t* d_tpl_ext = nullptr;   // n_tpl x 4: dxmin, dxmax, dymin, dymax
+    int32_t max_side = 0;           // largest template box side
+    int32_t win_slack = 3;          // deformation slack of the LDS window (pixels per side)
+    int32_t waves_per_cu = 12;      // persistent grid size for the batched engine
+    int64_t* d_scr_off = nullptr;   // n: offset of each subset's scratch
+    double* d_p0 = nullptr;         // n x 12
+    double* d_p = nullptr;          // n x 12 final warps
+    dic_subset_result* d_res = nullptr;
+    double* d_hist = nullptr;       // n x max_it x 4 (allocated on demand)
+    int32_t hist_cap = 0;
+    double* d_scratch = nullptr;    // plain path per-pixel arrays
+    size_t scratch_doubles = 0;
+    float* d_slab = nullptr;        // batched engine per-pixel slab (float4 per pixel)
+    size_t slab_floats = 0;
+    int64_t* d_slab_off = nullptr;  // n: first pixel of each subset in the slab
+    bool slab_off_ready = false;
+    int32_t* d_order_perm = nullptr;  // subsets sorted by template size (descending)
+    int32_t* d_queue = nullptr;       // work-queue counter
+    // host copies
+    std::vector<int64_t> h_tpl_start;
+    std::vector<int32_t> h_tpl_index;
+    std::vector<int64_t> h_scr_off;
+    int last_precision = -1;
+    int last_method = -1;
+    int last_max_it = 0;
+    bool last_history = false;
+};
+
+// prefilter.hip
+int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem);
+// icgn_plain.hip
+int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
+// icgn_batch.hip
+int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
+
+// device memory accounting
+int dev_alloc(Engine* e, void** p, size_t bytes);
+void dev_free(Engine* e, void* p, size_t bytes);
+
+}  // namespace geodic

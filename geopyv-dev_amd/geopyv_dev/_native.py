@@ -1,0 +1,320 @@
+"""ctypes binding of libgeodic.so (the C ABI declared in include/geodic.h).
+
+The shared library is the product: every solve in this package goes through
+it.  There is no CPU fallback; importing a GPU entry point without the built
+library raises.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgeodic.so")
+
+# status codes (include/geodic.h)
+OK = 0
+ERR_IMAGE_LOAD = 1
+ERR_FILE_NOT_FOUND = 2
+ERR_INVALID_INPUT = 3
+ERR_NOT_SOLVED = 8
+ERR_CANCELLED = 9
+ERR_SINGULAR_HESSIAN = 10
+ERR_SINGULAR_WARP = 11
+ERR_DEVICE = 20
+ERR_OUT_OF_MEMORY = 21
+
+PIX_U8, PIX_F64 = 0, 1
+MEM_HOST, MEM_DEVICE = 0, 1
+SHAPE = {"circle": 0, "square": 1, "semicircle": 2}
+METHOD = {"icgn": 0, "fagn": 1}
+PREC_PLAIN_F64, PREC_COMP_F32 = 0, 1
+
+EXPORTED = [
+    "dic_engine_create", "dic_engine_destroy", "dic_engine_sync", "dic_engine_timing", "dic_last_error",
+    "dic_engine_bytes_in_use", "dic_image_create", "dic_image_update", "dic_image_coef", "dic_image_info",
+    "dic_image_destroy", "dic_mask_local", "dic_mask_update", "dic_batch_create", "dic_batch_set_p0",
+    "dic_batch_solve", "dic_batch_results", "dic_batch_history", "dic_batch_subset_arrays",
+    "dic_batch_set_schedule", "dic_batch_device_p", "dic_batch_destroy",
+]
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("want_history", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("max_norm", ctypes.c_double),
+    ]
+
+
+RESULT_DTYPE = np.dtype([
+    ("p", np.float64, (12,)),
+    ("c_zncc", np.float64), ("c_znssd", np.float64), ("norm", np.float64),
+    ("eta_u", np.float64), ("eta_v", np.float64),
+    ("f_m", np.float64), ("delta_f", np.float64), ("sssig", np.float64), ("sigma", np.float64),
+    ("iterations", np.int32), ("converged", np.int32), ("quality_ok", np.int32),
+    ("status", np.int32), ("n_px", np.int32), ("order", np.int32),
+])
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+_lock = threading.Lock()
+
+
+class GeodicError(RuntimeError):
+    """A failed engine call; `status` is the dic_status code."""
+
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+def load():
+    """Load libgeodic.so and declare its prototypes."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libgeodic.so is not built ({LIB_PATH}); run __graft_entry__.build() "
+                              "or `make -C geopyv-dev_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.dic_last_error.restype = ctypes.c_char_p
+        L.dic_engine_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+        L.dic_engine_destroy.argtypes = [_vp]
+        L.dic_engine_destroy.restype = None
+        L.dic_engine_sync.argtypes = [_vp]
+        L.dic_engine_timing.argtypes = [_vp, _dp, _dp]
+        L.dic_engine_bytes_in_use.argtypes = [_vp]
+        L.dic_engine_bytes_in_use.restype = ctypes.c_int64
+        L.dic_image_create.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.c_int, ctypes.POINTER(_vp)]
+        L.dic_image_update.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
+        L.dic_image_coef.argtypes = [_vp, _vp]
+        L.dic_image_info.argtypes = [_vp, _i64p, _i64p, ctypes.POINTER(ctypes.c_int)]
+        L.dic_image_destroy.argtypes = [_vp]
+        L.dic_image_destroy.restype = None
+        L.dic_mask_local.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _i64p]
+        L.dic_mask_update.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp,
+                                      ctypes.c_int64, ctypes.c_int64, _vp, _i64p]
+        L.dic_batch_create.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int32, _vp, _vp,
+                                       ctypes.POINTER(_vp)]
+        L.dic_batch_set_p0.argtypes = [_vp, _vp, ctypes.c_int]
+        L.dic_batch_solve.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
+        L.dic_batch_results.argtypes = [_vp, _vp]
+        L.dic_batch_history.argtypes = [_vp, _vp]
+        L.dic_batch_subset_arrays.argtypes = [_vp, ctypes.c_int64, _vp, _vp]
+        L.dic_batch_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.dic_batch_device_p.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.dic_batch_destroy.argtypes = [_vp]
+        L.dic_batch_destroy.restype = None
+        _lib = L
+        return L
+
+
+def check(status):
+    if status != OK:
+        msg = load().dic_last_error().decode(errors="replace")
+        raise GeodicError(status, msg or f"geodic status {status}")
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# masks (host, no device needed)
+# ---------------------------------------------------------------------------
+def mask_local(shape, size):
+    L = load()
+    side = 2 * size + 1
+    coords = np.empty((side * side, 2), dtype=np.float64)
+    sm = np.empty((side, side), dtype=np.int32)
+    n = ctypes.c_int64()
+    st = L.dic_mask_local(SHAPE[shape], int(size), ptr(coords), ptr(sm), ctypes.byref(n))
+    if st:
+        raise RuntimeError(L.dic_last_error().decode())
+    return coords[:n.value].copy(), sm
+
+
+def mask_update(shape, size, centre, mask):
+    L = load()
+    mask = np.ascontiguousarray(mask, dtype=np.uint8)
+    side = 2 * size + 1
+    coords = np.empty((side * side, 2), dtype=np.float64)
+    n = ctypes.c_int64()
+    st = L.dic_mask_update(SHAPE[shape], int(size), float(centre[0]), float(centre[1]), ptr(mask),
+                           mask.shape[0], mask.shape[1], ptr(coords), ctypes.byref(n))
+    if st:
+        raise RuntimeError(L.dic_last_error().decode())
+    return coords[:n.value].copy()
+
+
+# ---------------------------------------------------------------------------
+# device objects
+# ---------------------------------------------------------------------------
+class Engine:
+    """One engine (HIP stream + caches) per device, shared by the package."""
+
+    _engines = {}
+
+    def __init__(self, device=0):
+        L = load()
+        h = _vp()
+        check(L.dic_engine_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def get(cls, device=0):
+        with _lock:
+            e = cls._engines.get(device)
+        if e is None:
+            e = cls(device)
+            with _lock:
+                cls._engines[device] = e
+        return e
+
+    def sync(self):
+        check(load().dic_engine_sync(self.handle))
+
+    def timing(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(load().dic_engine_timing(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def bytes_in_use(self):
+        return load().dic_engine_bytes_in_use(self.handle)
+
+
+class DeviceImage:
+    """Device-resident B-spline coefficients of one image."""
+
+    def __init__(self, engine, pixels=None, device_ptr=None, shape=None, pix_type=None, border=20):
+        L = load()
+        self.engine = engine
+        h = _vp()
+        if device_ptr is not None:
+            hh, ww = shape
+            check(L.dic_image_create(engine.handle, ctypes.c_void_p(device_ptr), pix_type, MEM_DEVICE, hh, ww,
+                                     int(border), ctypes.byref(h)))
+        else:
+            pixels = np.ascontiguousarray(pixels)
+            if pixels.dtype == np.uint8:
+                pt = PIX_U8
+            else:
+                pixels = np.ascontiguousarray(pixels, dtype=np.float64)
+                pt = PIX_F64
+            hh, ww = pixels.shape
+            check(L.dic_image_create(engine.handle, ptr(pixels), pt, MEM_HOST, hh, ww, int(border),
+                                     ctypes.byref(h)))
+        self.handle = h
+        self.shape = (int(hh), int(ww))
+        self.border = int(border)
+
+    def update(self, pixels=None, device_ptr=None, pix_type=None):
+        L = load()
+        if device_ptr is not None:
+            check(L.dic_image_update(self.handle, ctypes.c_void_p(device_ptr), pix_type, MEM_DEVICE))
+        else:
+            pixels = np.ascontiguousarray(pixels)
+            pt = PIX_U8 if pixels.dtype == np.uint8 else PIX_F64
+            if pt == PIX_F64:
+                pixels = np.ascontiguousarray(pixels, dtype=np.float64)
+            check(L.dic_image_update(self.handle, ptr(pixels), pt, MEM_HOST))
+
+    def coef(self):
+        h, w = self.shape
+        b = self.border
+        out = np.empty((h + 2 * b, w + 2 * b), dtype=np.float64)
+        check(load().dic_image_coef(self.handle, ptr(out)))
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().dic_image_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """A set of subsets solved together (dic_batch_*)."""
+
+    def __init__(self, engine, coords, order, tpl_index, tpl_start, tpl_xy):
+        L = load()
+        self.engine = engine
+        coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 2)
+        n = len(coords)
+        orders = np.full(n, int(order), dtype=np.int32)
+        tpl_index = np.ascontiguousarray(tpl_index, dtype=np.int32)
+        tpl_start = np.ascontiguousarray(tpl_start, dtype=np.int64)
+        tpl_xy = np.ascontiguousarray(tpl_xy, dtype=np.int16).reshape(-1, 2)
+        h = _vp()
+        check(L.dic_batch_create(engine.handle, n, ptr(coords), ptr(orders), ptr(tpl_index),
+                                 len(tpl_start) - 1, ptr(tpl_start), ptr(tpl_xy), ctypes.byref(h)))
+        self.handle = h
+        self.n = n
+        self.order = int(order)
+        self.tpl_index = tpl_index
+        self.tpl_start = tpl_start
+        self._keep = (coords, orders, tpl_xy)
+        self.max_iterations = 0
+
+    def set_p0(self, p0):
+        p0 = np.ascontiguousarray(p0, dtype=np.float64).reshape(self.n, 12)
+        check(load().dic_batch_set_p0(self.handle, ptr(p0), MEM_HOST))
+
+    def set_p0_device(self, device_ptr):
+        check(load().dic_batch_set_p0(self.handle, ctypes.c_void_p(device_ptr), MEM_DEVICE))
+
+    def set_schedule(self, win_slack, waves_per_cu):
+        check(load().dic_batch_set_schedule(self.handle, int(win_slack), int(waves_per_cu)))
+
+    def solve(self, f_img, g_img, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50, tolerance=0.75,
+              max_norm=1e-5, history=False):
+        prm = SolveParams(METHOD[method], int(precision), int(max_iterations), 1 if history else 0,
+                          float(tolerance), float(max_norm))
+        self.max_iterations = int(max_iterations)
+        check(load().dic_batch_solve(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm)))
+
+    def results(self):
+        out = np.zeros(self.n, dtype=RESULT_DTYPE)
+        check(load().dic_batch_results(self.handle, ptr(out)))
+        return out
+
+    def history(self):
+        cap = max(self.max_iterations, 1)
+        out = np.zeros((self.n, cap, 4), dtype=np.float64)
+        check(load().dic_batch_history(self.handle, ptr(out)))
+        return out
+
+    def subset_arrays(self, s):
+        t = self.tpl_index[s]
+        n = int(self.tpl_start[t + 1] - self.tpl_start[t])
+        f = np.empty(n)
+        g = np.empty((n, 2))
+        check(load().dic_batch_subset_arrays(self.handle, int(s), ptr(f), ptr(g)))
+        return f, g
+
+    def device_p(self):
+        p = _vp()
+        check(load().dic_batch_device_p(self.handle, ctypes.byref(p)))
+        return p.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().dic_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

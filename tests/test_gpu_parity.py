@@ -1,0 +1,172 @@
+"""GPU parity: the HIP path (through libgeodic.so) against the FP64 oracle.
+
+Bars (stated per test): coefficient arrays rtol 1e-8 (Tier B of the
+reference, image.rs:21-23); pixel lists bit-exact; plain FP64 drop-in
+displacements within 1e-4 px, strains within 1e-6, identical convergence
+flags and iteration counts; the FP32-interpolation engine within 1e-4 px and
+the strain bound stated in its test.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+import geopyv_dev as gp
+from geopyv_dev import _native, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _speckle(shape, n, sigma, seed, field, noise=2.0):
+    sp = workloads.Speckle(shape, n, sigma, seed)
+    return sp.reference(noise), sp.target(field, noise)
+
+
+@pytest.fixture(scope="module")
+def pair0():
+    w = workloads.single_subset_pair(seed=0)
+    ref = w["ref"].astype(np.float64)
+    tar = w["tar"].astype(np.float64)
+    return dict(ref=ref, tar=tar, Cf=O.get_c(ref, 20), Cg=O.get_c(tar, 20), w=w)
+
+
+@pytest.mark.parametrize("shape,border", [((512, 512), 20), ((37, 53), 5), ((64, 81), 3), ((20, 25), 5)])
+def test_prefilter_matches_oracle(shape, border):
+    rng = np.random.default_rng(7)
+    img = rng.uniform(0, 255, shape)
+    I = gp.Image(image_gs=img, border=border)
+    C = I.coef
+    Co = O.get_c(img, border)
+    assert C.shape == Co.shape
+    np.testing.assert_allclose(C, Co, rtol=1e-8, atol=1e-8 * 255)
+    assert np.max(np.abs(C - Co)) < 1e-10
+
+
+def test_prefilter_u8_equals_f64():
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 256, (200, 300)).astype(np.uint8)
+    e = gp._engine()
+    a = _native.DeviceImage(e, pixels=img, border=20).coef()
+    b = _native.DeviceImage(e, pixels=img.astype(np.float64), border=20).coef()
+    assert np.array_equal(a, b)
+
+
+def test_prefilter_constant_image():
+    # image.rs:538-550: constant image -> constant C
+    I = gp.Image(image_gs=np.full((20, 25), 77.0), border=5)
+    assert np.max(np.abs(I.coef - 77.0)) < 1e-8 * 77.0
+
+
+def test_prefilter_config_sizes():
+    # every padded length of the workloads: 2088 = 2^3·3^2·29, 1064 = 2^3·7·19
+    rng = np.random.default_rng(11)
+    for shape in [(1024, 1024), (2048, 64)]:
+        img = rng.uniform(0, 255, shape)
+        C = gp.Image(image_gs=img, border=20).coef
+        np.testing.assert_allclose(C, O.get_c(img, 20), rtol=1e-8, atol=1e-9)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("solver", ["icgn", "fagn"])
+def test_single_subset_drop_in(pair0, order, solver):
+    ref, tar = pair0["ref"], pair0["tar"]
+    f = gp.Image(image_gs=ref, border=20)
+    g = gp.Image(image_gs=tar, border=20)
+    m = gp.Mask(mask_type="local", shape="circle", size=25)
+    coord = [256.0, 256.0]
+    s = gp.Subset(coord, m, f, g, subset_order=order)
+    s.solve(solver=solver, max_norm=1e-5, max_iterations=50)
+    o = O.subset_solve(pair0["Cf"], pair0["Cg"], ref.shape, 20, coord, m.coords, order, method=solver,
+                       max_norm=1e-5, max_iterations=50, want_arrays=True)
+    print(solver, order, s.p, o["p"], s.iterations, o["iterations"])
+    assert s.n_px == o["n_px"] == 1961
+    np.testing.assert_allclose(s.f, o["f"], rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(s.grad_f, o["grad_f"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose([s.f_m, s.delta_f, s.sssig, s.sigma_intensity],
+                               [o["f_m"], o["delta_f"], o["sssig"], o["sigma"]], rtol=1e-10)
+    assert s.converged == o["converged"]
+    assert s.iterations == o["iterations"]
+    np.testing.assert_allclose(s.p[:2], o["p"][:2], atol=1e-4)
+    if order >= 1:
+        np.testing.assert_allclose(s.p[2:6], o["p"][2:6], atol=1e-6)
+    np.testing.assert_allclose(s.c_zncc, o["c_zncc"], atol=1e-9)
+    np.testing.assert_allclose([s.eta_u, s.eta_v], [o["eta_u"], o["eta_v"]], rtol=1e-6, atol=1e-9)
+    assert len(s.history) == len(o["history"])
+    for a, b in zip(s.history, o["history"]):
+        assert a[0] == b[0]
+        np.testing.assert_allclose(a[1:], b[1:], rtol=1e-5, atol=1e-9)
+    # the translation is recovered
+    assert abs(s.p[0] - 0.37) < 0.02 and abs(s.p[1] + 0.52) < 0.02
+
+
+def test_non_convergence_and_quality(pair0):
+    f = gp.Image(image_gs=pair0["ref"], border=20)
+    g = gp.Image(image_gs=pair0["tar"], border=20)
+    m = gp.Mask(mask_type="local", shape="circle", size=25)
+    s = gp.Subset([256.0, 256.0], m, f, g)
+    s.solve(max_norm=1e-8, max_iterations=1, tolerance=0.0)
+    assert s.iterations == 1 and not s.converged and not s.quality_ok
+    s.solve(tolerance=2.0)
+    assert s.converged and not s.quality_ok
+
+
+def _mesh_case(n_side, seed=5, shape=(640, 640)):
+    w = workloads.uniform_mesh_pair(seed=seed, n_side=n_side, shape=shape)
+    ref = w["ref"].astype(np.float64)
+    tar = w["tar"].astype(np.float64)
+    return w, ref, tar
+
+
+@pytest.fixture(scope="module")
+def mesh_small():
+    w, ref, tar = _mesh_case(24)
+    Cf, Cg = O.get_c(ref, 20), O.get_c(tar, 20)
+    m = gp.Mask(mask_type="local", shape="circle", size=20)
+    xy = m.coords
+    n = len(w["coords"])
+    orc = O.batch_solve(Cf, Cg, ref.shape, 20, w["coords"], np.zeros(n, np.int32), xy, np.array([0, len(xy)]),
+                        np.full(n, 1, np.int32), w["p0"], max_norm=1e-5, max_iterations=50)
+    return dict(w=w, ref=ref, tar=tar, m=m, orc=orc)
+
+
+def test_batch_plain_f64_matches_oracle(mesh_small):
+    w, m, orc = mesh_small["w"], mesh_small["m"], mesh_small["orc"]
+    f = gp.Image(image_gs=mesh_small["ref"], border=20)
+    g = gp.Image(image_gs=mesh_small["tar"], border=20)
+    r = gp.solve_subsets(f, g, w["coords"], m, p0=w["p0"], precision="f64", max_norm=1e-5)
+    assert np.all(r["status"] == 0) and np.all(orc["status"] == 0)
+    du = np.abs(r["p"][:, :2] - orc["p"][:, :2]).max()
+    ds = np.abs(r["p"][:, 2:6] - orc["p"][:, 2:6]).max()
+    print("plain f64: max |du|", du, "max |dstrain|", ds)
+    assert du < 1e-4 and ds < 1e-6
+    assert np.array_equal(r["converged"], orc["converged"])
+    assert np.array_equal(r["iterations"], orc["iterations"])
+    np.testing.assert_allclose(r["c_zncc"], orc["c_zncc"], atol=1e-9)
+
+
+def test_batch_f32_engine_matches_oracle(mesh_small):
+    w, m, orc = mesh_small["w"], mesh_small["m"], mesh_small["orc"]
+    f = gp.Image(image_gs=mesh_small["ref"], border=20)
+    g = gp.Image(image_gs=mesh_small["tar"], border=20)
+    r = gp.solve_subsets(f, g, w["coords"], m, p0=w["p0"], precision="f32", max_norm=1e-5)
+    assert np.all(r["status"] == 0)
+    du = np.abs(r["p"][:, :2] - orc["p"][:, :2]).max()
+    ds = np.abs(r["p"][:, 2:6] - orc["p"][:, 2:6]).max()
+    it_mismatch = np.mean(r["iterations"] != orc["iterations"])
+    print("f32 engine: max |du|", du, "max |dstrain|", ds, "iteration mismatch", it_mismatch,
+          "conv", r["converged"].mean(), orc["converged"].mean())
+    assert du < 1e-4
+    assert ds < 1e-5
+    assert np.array_equal(r["converged"], orc["converged"])
+    np.testing.assert_allclose(r["c_zncc"], orc["c_zncc"], atol=1e-6)
+
+
+def test_batch_deterministic_alone_and_in_mix(mesh_small):
+    w, m = mesh_small["w"], mesh_small["m"]
+    f = gp.Image(image_gs=mesh_small["ref"], border=20)
+    g = gp.Image(image_gs=mesh_small["tar"], border=20)
+    full = gp.solve_subsets(f, g, w["coords"], m, p0=w["p0"], precision="f32")
+    for k in (0, 7, len(w["coords"]) - 1):
+        alone = gp.solve_subsets(f, g, w["coords"][k:k + 1], m, p0=w["p0"][k:k + 1], precision="f32")
+        assert np.array_equal(alone["p"][0], full["p"][k])
+        assert alone["iterations"][0] == full["iterations"][k]
