@@ -1,0 +1,189 @@
+"""Benchmark: ICGN subsets/s of the FP32-interpolation batched engine against
+the plain FP64 drop-in, on one MI355X.
+
+Workload (one step = one frame): a 2048x2048 8-bit speckle pair (50 000
+Gaussian speckles, sigma 2.5 px, noise sigma 2 grey levels) under an affine
+field (u_x = 1 %, v_y = -0.3 %, shear u_y = 0.5 %); the reference image's
+B-spline coefficients stay resident, and each frame prefilters the target
+(GPU FFT deconvolution) and solves 40 000 circle subsets of radius 20 on a
+10 px grid (first-order warp, ||dp|| < 1e-5, at most 50 iterations) from
+p0 = the true displacement rounded to whole pixels.  Pixels are resident in
+HBM before the timed region.
+
+Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle
+(a C++ restatement of the reference's CPU path) timed on a bounded sample of
+the same subsets with the host's cores; the same sample checks the GPU
+results against it.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "geopyv-dev_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=1500)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU "
+              "path in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device buffers for the resident input pixels
+    import geopyv_dev as gp
+    from geopyv_dev import _native, workloads
+
+    t0 = time.time()
+    w = workloads.uniform_mesh_pair(seed=1)
+    log(f"workload generated in {time.time() - t0:.1f}s")
+    ref, tar = w["ref"], w["tar"]
+    coords, p0, truth = w["coords"], w["p0"], w["truth"]
+    n = len(coords)
+    radius = w["radius"]
+    h, wd = ref.shape
+    border = 20
+
+    eng = gp._engine()
+    f_img = _native.DeviceImage(eng, pixels=ref, border=border)
+    dev_tar = torch.from_numpy(tar).cuda()
+    torch.cuda.synchronize()
+    g_img = _native.DeviceImage(eng, device_ptr=dev_tar.data_ptr(), shape=tar.shape, pix_type=_native.PIX_U8,
+                                border=border)
+    mask = gp.Mask(mask_type="local", shape="circle", size=radius)
+    xy = np.rint(mask.coords).astype(np.int16)
+    batch = _native.Batch(eng, coords, 1, np.zeros(n, np.int32), np.array([0, len(xy)], np.int64), xy)
+    batch.set_p0(p0)
+
+    def frame(precision):
+        g_img.update(device_ptr=dev_tar.data_ptr(), pix_type=_native.PIX_U8)
+        batch.solve(f_img, g_img, method="icgn", precision=precision, max_iterations=50, tolerance=0.75,
+                    max_norm=1e-5)
+        eng.sync()
+
+    # ---- FP32-interpolation batched engine (the measured value) ----
+    for _ in range(args.warmup):
+        frame(_native.PREC_COMP_F32)
+    lat, pre_ms, sol_ms = [], [], []
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        frame(_native.PREC_COMP_F32)
+        lat.append((time.perf_counter() - t) * 1e3)
+        a, b = eng.timing()
+        pre_ms.append(a)
+        sol_ms.append(b)
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - tb
+    res32 = batch.results()
+    log(f"f32 engine: {np.mean(lat):.3f} ms/frame, solve {np.mean(sol_ms):.3f} ms, prefilter {np.mean(pre_ms):.3f} ms")
+
+    # ---- plain FP64 drop-in on the same frames ----
+    plain_lat, plain_sol = [], []
+    frame(_native.PREC_PLAIN_F64)
+    for _ in range(args.plain_steps):
+        t = time.perf_counter()
+        frame(_native.PREC_PLAIN_F64)
+        plain_lat.append((time.perf_counter() - t) * 1e3)
+        plain_sol.append(eng.timing()[1])
+    res64 = batch.results()
+    log(f"fp64 plain: {np.mean(plain_lat):.3f} ms/frame, solve {np.mean(plain_sol):.3f} ms")
+
+    ms_per_step = total_s * 1e3 / args.steps
+    value = n / (total_s / args.steps)
+    it_mean = float(np.mean(res32["iterations"]))
+    # algorithmic bytes per subset: the coefficient window once for setup and
+    # once per iteration (FP32), p0 in and the result out
+    m = 6
+    wx = 2 * int(np.ceil(radius * (1 + 0.010 + 0.005))) + 6
+    wy = 2 * int(np.ceil(radius * (1 + 0.0 + 0.003))) + 6
+    bytes_per_subset = 4 * wx * wy * (1 + it_mean) + 24 * m + 64
+    sol_avg = float(np.mean(sol_ms))
+    achieved = n * bytes_per_subset / (sol_avg * 1e-3) / 1e9
+
+    out = {
+        "metric": "ICGN subsets/sec (FP32-comp vs FP64 plain, <=1e-4 px); p50/p99 frame latency",
+        "value": round(value, 1),
+        "unit": "subsets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "uniform mesh: 2048x2048 speckle pair, 40000 circle subsets r=20 on a 10 px grid, "
+                               "order-1 warp, affine field, target prefiltered per frame",
+                   "subsets": n, "radius": radius, "image": [h, wd], "order": 1, "max_norm": 1e-5},
+        "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                             "p99": round(float(np.percentile(lat, 99)), 4)},
+        "frames_per_s": round(1e3 / ms_per_step, 3),
+        "kernel_ms": {"prefilter": round(float(np.mean(pre_ms)), 4), "icgn_batch": round(sol_avg, 4),
+                      "icgn_plain_f64": round(float(np.mean(plain_sol)), 4)},
+        "fp64_plain": {"subsets_per_s": round(n / (np.mean(plain_lat) * 1e-3), 1),
+                       "ms_per_frame": round(float(np.mean(plain_lat)), 4)},
+        "speedup_vs_fp64_plain": round(float(np.mean(plain_lat) / np.mean(lat)), 3),
+        "iterations_mean": round(it_mean, 3),
+        "converged_fraction": round(float(np.mean(res32["converged"])), 5),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_subset": round(bytes_per_subset, 1), "kernel": "icgn_batch_kernel<6>"},
+        "error_vs_truth_px": {"f32": float(np.abs(res32["p"][:, :2] - truth[:, :2]).max()),
+                              "fp64_plain": float(np.abs(res64["p"][:, :2] - truth[:, :2]).max())},
+    }
+
+    # ---- CPU baseline + parity on a bounded sample ----
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        idx = np.linspace(0, n - 1, min(args.cpu_sample, n)).astype(np.int64)
+        Cf = O.get_c(ref.astype(np.float64), border, threads)
+        Cg = O.get_c(tar.astype(np.float64), border, threads)
+        t = O.wall()
+        orc = O.batch_solve(Cf, Cg, ref.shape, border, coords[idx], np.zeros(len(idx), np.int32),
+                            mask.coords, np.array([0, len(xy)]), np.full(len(idx), 1, np.int32), p0[idx],
+                            max_norm=1e-5, max_iterations=50, threads=threads)
+        cpu_s = O.wall() - t
+        out["cpu_baseline"] = {"value": round(len(idx) / cpu_s, 2), "unit": "subsets/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{len(idx)} of the {n} subsets (evenly strided), FP64 oracle with "
+                                         f"OpenMP over subsets, {cpu_s:.1f} s"}
+        d32 = res32[idx]
+        d64 = res64[idx]
+        out["error_vs_oracle"] = {
+            "f32_max_du_px": float(np.abs(d32["p"][:, :2] - orc["p"][:, :2]).max()),
+            "f32_max_dstrain": float(np.abs(d32["p"][:, 2:6] - orc["p"][:, 2:6]).max()),
+            "f32_iterations_match": float(np.mean(d32["iterations"] == orc["iterations"])),
+            "fp64_max_du_px": float(np.abs(d64["p"][:, :2] - orc["p"][:, :2]).max()),
+            "fp64_max_dstrain": float(np.abs(d64["p"][:, 2:6] - orc["p"][:, 2:6]).max()),
+            "fp64_iterations_match": float(np.mean(d64["iterations"] == orc["iterations"])),
+            "converged_match": float(np.mean(d64["converged"] == orc["converged"])),
+            "sample": len(idx),
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -6,7 +6,9 @@
 // 130-131), so out-of-range queries saturate to the boundary value.
 #pragma once
 
+#ifndef GEODIC_SIMT
 #include <hip/hip_runtime.h>
+#endif
 #include <cstdint>
 
 namespace geodic {

@@ -2,7 +2,9 @@
 // usable from device code (one lane or all lanes redundantly).
 #pragma once
 
+#ifndef GEODIC_SIMT
 #include <hip/hip_runtime.h>
+#endif
 
 namespace geodic {
 

@@ -1,0 +1,164 @@
+// Runs the batched-engine kernel (icgn_batch_kernel.h) through the CPU SIMT
+// emulation on a synthetic speckle pair and compares every subset with the
+// FP64 oracle.  Built with AddressSanitizer/UBSan by tests/simt/Makefile.
+//
+//   simt_driver ORDER RADIUS NSIDE MAXIT [WIN_SLACK]
+// exit 0 when every subset matches (|du|,|dv| < 1e-4 px), 1 otherwise.
+#include "simt_hip.h"
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "../../geopyv-dev_amd/csrc/icgn_batch_kernel.h"
+
+extern "C" {
+void orc_get_c(const double* img, int rows, int cols, int border, double* out, int threads);
+int orc_mask_local(int shape, int size, double* coords, int32_t* subset_mask);
+struct OrcResult {
+    double p[12];
+    double c_zncc, c_znssd, norm, eta_u, eta_v, f_m, delta_f, sssig, sigma;
+    int32_t iterations, converged, quality_ok, status, n_px, order;
+};
+int orc_batch_solve(const double* fcoef, const double* gcoef, int h, int w, int border, int64_t n_sub,
+                    const double* coords, const int32_t* tpl_index, const double* tpl_xy, const int64_t* tpl_start,
+                    const int32_t* orders, const double* p0, int method, double tolerance, double max_norm,
+                    int max_iterations, OrcResult* out, int threads);
+}
+
+static double speckle(double x, double y, const std::vector<double>& cx, const std::vector<double>& cy) {
+    double s = 0.0;
+    for (size_t k = 0; k < cx.size(); ++k) {
+        const double dx = x - cx[k], dy = y - cy[k];
+        s += std::exp(-(dx * dx + dy * dy) / 8.0);
+    }
+    return 200.0 * s;
+}
+
+int main(int argc, char** argv) {
+    const int order = argc > 1 ? atoi(argv[1]) : 1;
+    const int radius = argc > 2 ? atoi(argv[2]) : 10;
+    const int nside = argc > 3 ? atoi(argv[3]) : 3;
+    const int max_it = argc > 4 ? atoi(argv[4]) : 50;
+    const int slack = argc > 5 ? atoi(argv[5]) : 3;
+    const int H = 96, W = 96, B = 20;
+    const int HP = H + 2 * B, WP = W + 2 * B;
+    uint64_t st = 12345;
+    auto rnd = [&]() {
+        st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+        return (double)(st >> 11) / 9007199254740992.0;
+    };
+    std::vector<double> cx(300), cy(300);
+    for (int k = 0; k < 300; ++k) {
+        cx[k] = rnd() * W;
+        cy[k] = rnd() * H;
+    }
+    std::vector<double> f((size_t)H * W), g((size_t)H * W);
+    const double u = 0.37, v = -0.52, ux = 0.004, vy = -0.003;
+    for (int i = 0; i < H; ++i)
+        for (int j = 0; j < W; ++j) {
+            f[(size_t)i * W + j] = speckle(j, i, cx, cy);
+            // target(x) = f(X) where x = X + u + ux (X - 48), y = Y + v + vy (Y - 48)
+            const double X = (j - u + 48.0 * ux) / (1.0 + ux);
+            const double Y = (i - v + 48.0 * vy) / (1.0 + vy);
+            g[(size_t)i * W + j] = speckle(X, Y, cx, cy);
+        }
+    std::vector<double> Cf((size_t)HP * WP), Cg((size_t)HP * WP);
+    orc_get_c(f.data(), H, W, B, Cf.data(), 1);
+    orc_get_c(g.data(), H, W, B, Cg.data(), 1);
+    std::vector<float> Cf32(Cf.begin(), Cf.end()), Cg32(Cg.begin(), Cg.end());
+
+    // template and subsets
+    const int side = 2 * radius + 1;
+    std::vector<double> tc((size_t)side * side * 2);
+    const int npx = orc_mask_local(0, radius, tc.data(), nullptr);
+    std::vector<int32_t> packed(npx);
+    int32_t ext[4] = {0, 0, 0, 0};
+    for (int i = 0; i < npx; ++i) {
+        const int dx = (int)tc[2 * i], dy = (int)tc[2 * i + 1];
+        packed[i] = (int32_t)(((uint32_t)(uint16_t)dx) | ((uint32_t)(uint16_t)dy << 16));
+        if (i == 0) {
+            ext[0] = ext[1] = dx;
+            ext[2] = ext[3] = dy;
+        }
+        ext[0] = std::min(ext[0], dx);
+        ext[1] = std::max(ext[1], dx);
+        ext[2] = std::min(ext[2], dy);
+        ext[3] = std::max(ext[3], dy);
+    }
+    const int n = nside * nside;
+    std::vector<double> coords(2 * n), p0(12 * n, 0.0), pout(12 * n, 0.0);
+    for (int a = 0; a < nside; ++a)
+        for (int b = 0; b < nside; ++b) {
+            const int s = a * nside + b;
+            coords[2 * s] = 30.0 + 18.0 * b + (s % 2 ? 0.25 : 0.0);
+            coords[2 * s + 1] = 30.0 + 18.0 * a;
+        }
+    std::vector<int32_t> tpl_index(n, 0), perm(n), orders(n, order);
+    for (int s = 0; s < n; ++s) perm[s] = s;
+    std::vector<int64_t> tpl_start = {0, npx}, slab_off(n);
+    for (int s = 0; s < n; ++s) slab_off[s] = (int64_t)s * npx;
+    std::vector<float4> slab((size_t)n * npx);
+    std::vector<dic_subset_result> res(n);
+    const int M = order == 0 ? 2 : (order == 1 ? 6 : 12);
+    std::vector<double> hist((size_t)n * max_it * 4 + 4, 0.0);
+
+    geodic::batch::BatchArgs A{};
+    A.fC = Cf32.data();
+    A.gC = Cg32.data();
+    A.wp = WP;
+    A.border = B;
+    A.w = W;
+    A.h = H;
+    A.wmax = W - 1e-9;
+    A.hmax = H - 1e-9;
+    A.n_sub = n;
+    A.coords = coords.data();
+    A.tpl_index = tpl_index.data();
+    A.tpl_start = tpl_start.data();
+    A.tpl_xy = packed.data();
+    A.tpl_ext = ext;
+    A.perm = perm.data();
+    A.slab = slab.data();
+    A.slab_off = slab_off.data();
+    A.p0 = p0.data();
+    A.p_out = pout.data();
+    A.res = res.data();
+    A.hist = hist.data();
+    A.hist_cap = max_it;
+    A.max_it = max_it;
+    A.tol = 0.75;
+    A.max_norm = 1e-5;
+    const int wside = (ext[1] - ext[0] + 1) + 6 + 2 * slack;
+    A.win_cap = wside * wside;
+    const size_t lds = sizeof(float) * A.win_cap;
+    if (M == 2)
+        simt::launch(n, lds, [&] { geodic::batch::icgn_batch_kernel<2>(A); });
+    else if (M == 6)
+        simt::launch(n, lds, [&] { geodic::batch::icgn_batch_kernel<6>(A); });
+    else
+        simt::launch(n, lds, [&] { geodic::batch::icgn_batch_kernel<12>(A); });
+
+    std::vector<double> xy(2 * npx);
+    for (int i = 0; i < 2 * npx; ++i) xy[i] = tc[i];
+    std::vector<OrcResult> orc(n);
+    orc_batch_solve(Cf.data(), Cg.data(), H, W, B, n, coords.data(), tpl_index.data(), xy.data(), tpl_start.data(),
+                    orders.data(), p0.data(), 0, 0.75, 1e-5, max_it, orc.data(), 1);
+    int bad = 0;
+    double dumax = 0.0, dsmax = 0.0;
+    for (int s = 0; s < n; ++s) {
+        const double du = std::max(std::fabs(res[s].p[0] - orc[s].p[0]), std::fabs(res[s].p[1] - orc[s].p[1]));
+        double ds = 0.0;
+        for (int k = 2; k < M; ++k) ds = std::max(ds, std::fabs(res[s].p[k] - orc[s].p[k]));
+        dumax = std::max(dumax, du);
+        dsmax = std::max(dsmax, ds);
+        const bool ok = res[s].status == orc[s].status && du < 1e-4 && res[s].converged == orc[s].converged;
+        if (!ok) ++bad;
+        printf("s=%d st=%d/%d it=%d/%d conv=%d/%d u=%.8f/%.8f v=%.8f/%.8f zncc=%.9f/%.9f eta=%.3e/%.3e\n", s,
+               res[s].status, orc[s].status, res[s].iterations, orc[s].iterations, res[s].converged, orc[s].converged,
+               res[s].p[0], orc[s].p[0], res[s].p[1], orc[s].p[1], res[s].c_zncc, orc[s].c_zncc, res[s].eta_u,
+               orc[s].eta_u);
+    }
+    printf("order %d radius %d subsets %d: max|du| %.3e max|dstrain| %.3e mismatched %d\n", order, radius, n,
+           dumax, dsmax, bad);
+    return bad ? 1 : 0;
+}

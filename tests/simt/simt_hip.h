@@ -1,0 +1,125 @@
+// CPU emulation of the HIP device subset the engine's kernels use, for
+// checking kernel logic on the host (sanitizers, no GPU).  One block runs at
+// a time as 64 std::threads, one per lane; __syncthreads and every
+// cross-lane operation are barriers, so a shuffle reached by only part of a
+// wavefront deadlocks here instead of silently misbehaving.  Static
+// __shared__ arrays become function statics (one block at a time).
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define GEODIC_SIMT 1
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __shared__ static
+#define __launch_bounds__(...)
+#define GEODIC_BATCH_ATTRS
+#define GEODIC_PLAIN_ATTRS
+#define GEODIC_DYN_LDS(T, name) T* name = (T*)::simt::g_dyn_lds
+
+using std::max;
+using std::min;
+
+struct float4 {
+    float x, y, z, w;
+};
+struct double2 {
+    double x, y;
+};
+inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
+inline double2 make_double2(double x, double y) { return {x, y}; }
+
+namespace simt {
+constexpr int kLanes = 64;
+inline std::barrier<>* g_bar = nullptr;
+inline void* g_dyn_lds = nullptr;
+inline uint64_t g_xchg[kLanes];
+struct Idx {
+    int x = 0, y = 0, z = 0;
+};
+inline thread_local Idx t_thread;
+inline Idx g_block, g_blockdim{kLanes, 1, 1}, g_griddim{1, 1, 1};
+
+inline void sync() { g_bar->arrive_and_wait(); }
+
+template <class T>
+T xchg(T v, int src) {
+    static_assert(sizeof(T) <= 8, "exchange slot");
+    uint64_t bits = 0;
+    std::memcpy(&bits, &v, sizeof(T));
+    g_xchg[t_thread.x] = bits;
+    sync();
+    uint64_t o = g_xchg[src & (kLanes - 1)];
+    sync();
+    T r;
+    std::memcpy(&r, &o, sizeof(T));
+    return r;
+}
+
+// Runs `body` for blocks 0..grid-1, one block (64 lane threads) at a time.
+inline void launch(int grid, size_t dyn_lds_bytes, const std::function<void()>& body) {
+    std::vector<unsigned char> lds(dyn_lds_bytes + 64, 0xCD);
+    g_dyn_lds = lds.data();
+    g_griddim.x = grid;
+    for (int b = 0; b < grid; ++b) {
+        g_block.x = b;
+        std::barrier<> bar(kLanes);
+        g_bar = &bar;
+        std::vector<std::thread> th;
+        for (int l = 0; l < kLanes; ++l)
+            th.emplace_back([l, &body] {
+                t_thread.x = l;
+                body();
+            });
+        for (auto& t : th) t.join();
+    }
+}
+}  // namespace simt
+
+#define threadIdx (::simt::t_thread)
+#define blockIdx (::simt::g_block)
+#define blockDim (::simt::g_blockdim)
+#define gridDim (::simt::g_griddim)
+
+inline void __syncthreads() { simt::sync(); }
+template <class T>
+T __shfl_xor(T v, int off, int = 64) {
+    return simt::xchg(v, simt::t_thread.x ^ off);
+}
+template <class T>
+T __shfl(T v, int src, int = 64) {
+    return simt::xchg(v, src);
+}
+inline int __builtin_amdgcn_readfirstlane(int v) { return simt::xchg(v, 0); }
+inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline double __longlong_as_double(long long v) {
+    double d;
+    std::memcpy(&d, &v, 8);
+    return d;
+}
+inline int __double2loint(double d) {
+    uint64_t b;
+    std::memcpy(&b, &d, 8);
+    return (int)(uint32_t)b;
+}
+inline int __double2hiint(double d) {
+    uint64_t b;
+    std::memcpy(&b, &d, 8);
+    return (int)(uint32_t)(b >> 32);
+}
+inline double __hiloint2double(int hi, int lo) {
+    uint64_t b = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    double d;
+    std::memcpy(&d, &b, 8);
+    return d;
+}
