@@ -172,8 +172,29 @@ __device__ __forceinline__ double uni(double v) {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Butterfly sum returned in scalar registers.
-__device__ __forceinline__ double wave_sum_u(double v) { return uni(wave_sum(v)); }
+// Wavefront sum with DPP row operations (no LDS traffic): pairs and quads
+// by quad permutes, rows of 16 by row shifts, then the row broadcasts carry
+// rows 0-1 and 2-3 together; lane 63 ends with the total, which is read
+// into scalar registers.  The addition tree is fixed, so the result does not
+// depend on anything but the 64 inputs.  Needs every lane active.
+template <int CTRL, int ROW_MASK, bool BOUND_CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, BOUND_CTRL);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, BOUND_CTRL);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum_u(double v) {
+    v += dpp_move<0xb1, 0xf, true>(v);   // quad_perm [1,0,3,2]
+    v += dpp_move<0x4e, 0xf, true>(v);   // quad_perm [2,3,0,1]
+    v += dpp_move<0x114, 0xf, true>(v);  // row_shr:4
+    v += dpp_move<0x118, 0xf, true>(v);  // row_shr:8
+    v += dpp_move<0x142, 0xa, false>(v); // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc, false>(v); // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
 
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll

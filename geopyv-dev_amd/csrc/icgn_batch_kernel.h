@@ -149,14 +149,31 @@ struct Window {
     bool lds;
 };
 
-// Stage C[y0:y0+wh, x0:x0+ww] into LDS (row-coalesced).
+// Stage C[y0:y0+wh, x0:x0+ww] into LDS.  The window is walked as one flat
+// range (consecutive lanes take consecutive columns, so each row's loads
+// coalesce) in groups of eight elements per lane, so eight loads are in
+// flight before the first LDS store waits on its data.
 __device__ __forceinline__ void stage_window(float* win, const float* __restrict__ C, int64_t wp, const Window& W,
                                              int lane) {
-#pragma unroll 2
-    for (int r = 0; r < W.wh; ++r) {
-        const float* src = C + (int64_t)(W.y0 + r) * wp + W.x0;
-        float* dst = win + r * W.ww;
-        for (int c = lane; c < W.ww; c += 64) dst[c] = src[c];
+    const int total = W.ww * W.wh;
+    const float inv = 1.0f / (float)W.ww;
+    constexpr int G = 8;
+    for (int base = 0; base < total; base += 64 * G) {
+        float v[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int q = base + lane + 64 * k;
+            // exact row index for q < 2^22: the float quotient is off by far
+            // less than the 0.5/ww margin to the next integer
+            const int r = (int)(((float)q + 0.5f) * inv);
+            const int c = q - r * W.ww;
+            v[k] = (q < total) ? C[(int64_t)(W.y0 + r) * wp + W.x0 + c] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int q = base + lane + 64 * k;
+            if (q < total) win[q] = v[k];
+        }
     }
 }
 
@@ -201,42 +218,47 @@ __device__ __forceinline__ float eval32(const float* win, const float* __restric
     basis_f32(tx, b);
     const int lx = cx - W.x0, ly = cy - W.y0;
     const bool in = W.lds && lx >= 0 && ly >= 0 && lx + 6 <= W.ww && ly + 6 <= W.wh;
-    const float* base;
-    int stride;
-    if (in) {
-        base = win + ly * W.ww + lx;
-        stride = W.ww;
-    } else {
-        base = C + (int64_t)cy * wp + cx;
-        stride = (int)wp;
-    }
     float val = 0.f, ggx = 0.f, ggy = 0.f;
+    float da[6], db[6];
     if (GRAD) {
-        float da[6], db[6];
         dbasis_f32(ty, da);
         dbasis_f32(tx, db);
+    }
+    // Two explicit gather paths so each compiles to its own address space
+    // (ds_read from the staged window, global_load otherwise).
+    if (in) {
+        const float* base = win + ly * W.ww + lx;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            const float* row = base + r * stride;
+            const float* row = base + r * W.ww;
             float s = 0.f, t = 0.f;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                const float c = row[k];
-                s = fmaf(c, b[k], s);
-                t = fmaf(c, db[k], t);
+                s = fmaf(row[k], b[k], s);
+                if (GRAD) t = fmaf(row[k], db[k], t);
             }
             val = fmaf(a[r], s, val);
-            ggx = fmaf(a[r], t, ggx);
-            ggy = fmaf(da[r], s, ggy);
+            if (GRAD) {
+                ggx = fmaf(a[r], t, ggx);
+                ggy = fmaf(da[r], s, ggy);
+            }
         }
     } else {
+        const float* __restrict__ base = C + (int64_t)cy * wp + cx;
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            const float* row = base + r * stride;
-            float s = 0.f;
+            const float* row = base + r * wp;
+            float s = 0.f, t = 0.f;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) s = fmaf(row[k], b[k], s);
+            for (int k = 0; k < 6; ++k) {
+                s = fmaf(row[k], b[k], s);
+                if (GRAD) t = fmaf(row[k], db[k], t);
+            }
             val = fmaf(a[r], s, val);
+            if (GRAD) {
+                ggx = fmaf(a[r], t, ggx);
+                ggy = fmaf(da[r], s, ggy);
+            }
         }
     }
     gx = ggx;
@@ -252,7 +274,7 @@ __device__ __forceinline__ void hessian_rows(const float4* __restrict__ slab, co
     double acc[NE > 0 ? NE : 1];
 #pragma unroll
     for (int q = 0; q < NE; ++q) acc[q] = 0.0;
-    #pragma unroll 1
+    #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
         const int32_t o = offs[i];
         const float4 v = slab[i];
@@ -356,7 +378,7 @@ __global__ void GEODIC_BATCH_ATTRS icgn_batch_kernel(BatchArgs A) {
             if (W.lds) stage_window(win, A.fC, A.wp, W, lane);
             __syncthreads();
             double sf = 0.0, ss = 0.0;
-            #pragma unroll 1
+            #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
                 const int32_t o = offs[i];
                 float gx, gy;
@@ -371,7 +393,7 @@ __global__ void GEODIC_BATCH_ATTRS icgn_batch_kernel(BatchArgs A) {
             double sq = 0.0, sa_l = 0.0;
 #pragma unroll
             for (int j = 0; j < M; ++j) Aj[j] = Sj[j] = 0.0;
-            #pragma unroll 1
+            #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
                 const int32_t o = offs[i];
                 const float4 v = slab[i];
@@ -438,7 +460,7 @@ __global__ void GEODIC_BATCH_ATTRS icgn_batch_kernel(BatchArgs A) {
             double Bj[M];
 #pragma unroll
             for (int j = 0; j < M; ++j) Bj[j] = 0.0;
-            #pragma unroll 1
+            #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
                 const int32_t o = offs[i];
                 const double dx = (double)unpack_dx(o), dy = (double)unpack_dy(o);
@@ -524,7 +546,7 @@ __global__ void GEODIC_BATCH_ATTRS icgn_batch_kernel(BatchArgs A) {
         const double rg = it_done ? 1.0 / delta_g : qnan;
         const double gmf = it_done ? g_m : 0.0;
         double zz = 0.0, se = 0.0, sx = 0.0, sy = 0.0;
-        #pragma unroll 1
+        #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
             const float4 v = slab[i];
             const double z = ((double)v.x - f_m) * rf - ((double)v.w - gmf) * rg;
@@ -538,7 +560,7 @@ __global__ void GEODIC_BATCH_ATTRS icgn_batch_kernel(BatchArgs A) {
         const double e_bar = wave_sum_u(se) / dn;
         const double mx = wave_sum_u(sx) / dn, my = wave_sum_u(sy) / dn;
         double vx = 0.0, vy = 0.0;
-        #pragma unroll 1
+        #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
             const float4 v = slab[i];
             const double a = (double)v.y - mx, b = (double)v.z - my;
@@ -550,7 +572,7 @@ __global__ void GEODIC_BATCH_ATTRS icgn_batch_kernel(BatchArgs A) {
 #pragma unroll
         for (int q = 0; q < 2 * K; ++q) rs[q] = rv[q] = rp[q] = 0.0;
         for (int pass = 0; pass < 3; ++pass) {
-            #pragma unroll 1
+            #pragma unroll 4
             for (int i = lane; i < n; i += 64) {
                 const int32_t o = offs[i];
                 const float4 v = slab[i];

@@ -169,17 +169,17 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
         g[i] = 0.0;
         part += v;
     }
-    const double f_m = wave_sum(part) / dn;
+    const double f_m = wave_sum_u(part) / dn;
     part = 0.0;
     for (int64_t i = lane; i < n; i += 64) {
         const double d = f[i] - f_m;
         part += d * d;
     }
-    const double sq = wave_sum(part);
+    const double sq = wave_sum_u(part);
     const double delta_f = sqrt(sq);
     part = 0.0;
     for (int64_t i = lane; i < n; i += 64) part += 0.5 * (gxa[i] * gxa[i] + gya[i] * gya[i]);
-    const double sssig = wave_sum(part);
+    const double sssig = wave_sum_u(part);
     const double sigma = sqrt(sq / dn);
     if (lane == 0) {
         R->f_m = f_m;
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
             for (int a = 0; a < M; ++a)
 #pragma unroll
                 for (int b = a; b < M; ++b) {
-                    const double v = wave_sum(acc[q++]);
+                    const double v = wave_sum_u(acc[q++]);
                     if (lane == 0) {
                         sH[a * M + b] = v;
                         sH[b * M + a] = v;
@@ -258,13 +258,13 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
             g[i] = v;
             part += v;
         }
-        const double g_m = wave_sum(part) / dn;
+        const double g_m = wave_sum_u(part) / dn;
         part = 0.0;
         for (int64_t i = lane; i < n; i += 64) {
             const double d = g[i] - g_m;
             part += d * d;
         }
-        const double delta_g = sqrt(wave_sum(part));
+        const double delta_g = sqrt(wave_sum_u(part));
 
         double gz[M];
 #pragma unroll
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
             for (int a = 0; a < M; ++a)
 #pragma unroll
                 for (int b = a; b < M; ++b) {
-                    const double v = wave_sum(acc[q++]);
+                    const double v = wave_sum_u(acc[q++]);
                     if (lane == 0) {
                         sH[a * M + b] = v;
                         sH[b * M + a] = v;
@@ -322,8 +322,8 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
                 }
         }
 #pragma unroll
-        for (int j = 0; j < M; ++j) gz[j] = wave_sum(gz[j]);
-        zacc = wave_sum(zacc);
+        for (int j = 0; j < M; ++j) gz[j] = wave_sum_u(gz[j]);
+        zacc = wave_sum_u(zacc);
         __syncthreads();
         if (lane == 0) {
             int status = 0;
@@ -384,28 +384,28 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
     // warp-adequacy diagnostic on the final g (subset.rs:1201-1210)
     part = 0.0;
     for (int64_t i = lane; i < n; i += 64) part += g[i];
-    const double g_mf = wave_sum(part) / dn;
+    const double g_mf = wave_sum_u(part) / dn;
     part = 0.0;
     for (int64_t i = lane; i < n; i += 64) {
         const double d = g[i] - g_mf;
         part += d * d;
     }
-    const double delta_gf = sqrt(wave_sum(part));
+    const double delta_gf = sqrt(wave_sum_u(part));
     double se = 0.0, sx = 0.0, sy = 0.0;
     for (int64_t i = lane; i < n; i += 64) {
         se += (f[i] - f_m) / delta_f - (g[i] - g_mf) / delta_gf;
         sx += gxa[i];
         sy += gya[i];
     }
-    const double e_bar = wave_sum(se) / dn;
-    const double mx = wave_sum(sx) / dn, my = wave_sum(sy) / dn;
+    const double e_bar = wave_sum_u(se) / dn;
+    const double mx = wave_sum_u(sx) / dn, my = wave_sum_u(sy) / dn;
     double vx = 0.0, vy = 0.0;
     for (int64_t i = lane; i < n; i += 64) {
         const double a = gxa[i] - mx, b = gya[i] - my;
         vx += a * a;
         vy += b * b;
     }
-    const double sdx = sqrt(fmax(wave_sum(vx) / dn, 1e-18)), sdy = sqrt(fmax(wave_sum(vy) / dn, 1e-18));
+    const double sdx = sqrt(fmax(wave_sum_u(vx) / dn, 1e-18)), sdy = sqrt(fmax(wave_sum_u(vy) / dn, 1e-18));
     double rsum[2 * K], rvar[2 * K], rproj[2 * K];
 #pragma unroll
     for (int k = 0; k < 2 * K; ++k) rsum[k] = rvar[k] = rproj[k] = 0.0;
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
         }
     }
 #pragma unroll
-    for (int k = 0; k < 2 * K; ++k) rsum[k] = wave_sum(rsum[k]) / dn;
+    for (int k = 0; k < 2 * K; ++k) rsum[k] = wave_sum_u(rsum[k]) / dn;
     for (int64_t i = lane; i < n; i += 64) {
         const int32_t o = offs[i];
         const double x = (double)unpack_dx(o) + xc, y = (double)unpack_dy(o) + yc;
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
         }
     }
 #pragma unroll
-    for (int k = 0; k < 2 * K; ++k) rvar[k] = sqrt(fmax(wave_sum(rvar[k]) / dn, 1e-18));
+    for (int k = 0; k < 2 * K; ++k) rvar[k] = sqrt(fmax(wave_sum_u(rvar[k]) / dn, 1e-18));
     for (int64_t i = lane; i < n; i += 64) {
         const int32_t o = offs[i];
         const double x = (double)unpack_dx(o) + xc, y = (double)unpack_dy(o) + yc;
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
     double eu = 0.0, ev = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double a = wave_sum(rproj[k]) / dn, b = wave_sum(rproj[K + k]) / dn;
+        const double a = wave_sum_u(rproj[k]) / dn, b = wave_sum_u(rproj[K + k]) / dn;
         eu += a * a;
         ev += b * b;
     }

@@ -101,6 +101,37 @@ T __shfl(T v, int src, int = 64) {
     return simt::xchg(v, src);
 }
 inline int __builtin_amdgcn_readfirstlane(int v) { return simt::xchg(v, 0); }
+inline int __builtin_amdgcn_readlane(int v, int lane) { return simt::xchg(v, lane); }
+
+// DPP move on gfx9 encodings used by the kernels: quad_perm (0x00-0xff),
+// row_shl:n (0x101-0x10f), row_shr:n (0x111-0x11f), row_bcast:15 (0x142),
+// row_bcast:31 (0x143).  Lanes of rows outside row_mask keep `old`; a lane
+// whose source is outside its row reads 0 with bound_ctrl, else keeps old.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
+    (void)bank_mask;
+    const int lane = simt::t_thread.x;
+    const int row = lane >> 4, rl = lane & 15;
+    int from = -1;
+    if (ctrl < 0x100) {
+        from = (lane & ~3) | ((ctrl >> (2 * (lane & 3))) & 3);
+    } else if (ctrl >= 0x101 && ctrl <= 0x10f) {
+        const int n = ctrl & 15;
+        from = (rl + n < 16) ? lane + n : -1;
+    } else if (ctrl >= 0x111 && ctrl <= 0x11f) {
+        const int n = ctrl & 15;
+        from = (rl >= n) ? lane - n : -1;
+    } else if (ctrl == 0x142) {
+        from = row >= 1 ? 16 * row - 1 : -1;
+    } else if (ctrl == 0x143) {
+        from = row >= 2 ? 31 : -1;
+    } else {
+        std::abort();
+    }
+    const int v = simt::xchg(src, from < 0 ? lane : from);
+    if (!((row_mask >> row) & 1)) return old;
+    if (from < 0) return bound_ctrl ? 0 : old;
+    return v;
+}
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline double __longlong_as_double(long long v) {
     double d;
