@@ -294,7 +294,7 @@ __device__ __forceinline__ void hessian_rows(const float4* __restrict__ slab, co
     for (int a = A0; a < A1; ++a)
 #pragma unroll
         for (int b = a; b < M; ++b) {
-            const double v = wave_sum(acc[q]);
+            const double v = wave_sum_u(acc[q]);
             ++q;
             if (lane == 0) {
                 sH[a * M + b] = v;
