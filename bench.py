@@ -42,6 +42,12 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=40000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--waves-per-subset", type=int, default=0, help="batched-engine schedule (0 = automatic)")
+    ap.add_argument("--win-slack", type=int, default=3, help="batched-engine LDS window slack in pixels")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy (known HBM bytes) for counter scaling")
+    ap.add_argument("--engine", choices=["both", "f32"], default="both",
+                    help="f32: time only the batched engine (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU "
@@ -49,6 +55,15 @@ def main():
         sys.exit(2)
 
     import torch  # device buffers for the resident input pixels
+
+    if args.copy_calibration:
+        a = torch.ones(512 * 2 ** 20 // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 copies of 512 MiB")
+        return
     import geopyv_dev as gp
     from geopyv_dev import _native, workloads
 
@@ -72,6 +87,7 @@ def main():
     xy = np.rint(mask.coords).astype(np.int16)
     batch = _native.Batch(eng, coords, 1, np.zeros(n, np.int32), np.array([0, len(xy)], np.int64), xy)
     batch.set_p0(p0)
+    batch.set_schedule(args.win_slack, args.waves_per_subset)
 
     def frame(precision):
         g_img.update(device_ptr=dev_tar.data_ptr(), pix_type=_native.PIX_U8)
@@ -99,12 +115,20 @@ def main():
 
     # ---- plain FP64 drop-in on the same frames ----
     plain_lat, plain_sol = [], []
-    frame(_native.PREC_PLAIN_F64)
+    if args.engine == "f32":
+        args.plain_steps = 0
+        args.no_cpu = True
+    else:
+        frame(_native.PREC_PLAIN_F64)
     for _ in range(args.plain_steps):
         t = time.perf_counter()
         frame(_native.PREC_PLAIN_F64)
         plain_lat.append((time.perf_counter() - t) * 1e3)
         plain_sol.append(eng.timing()[1])
+    if not plain_lat:
+        log(json.dumps({"f32_ms_per_frame": float(np.mean(lat)), "f32_solve_ms": float(np.mean(sol_ms)),
+                        "prefilter_ms": float(np.mean(pre_ms))}))
+        return
     res64 = batch.results()
     log(f"fp64 plain: {np.mean(plain_lat):.3f} ms/frame, solve {np.mean(plain_sol):.3f} ms")
 

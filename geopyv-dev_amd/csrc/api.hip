@@ -362,12 +362,14 @@ int dic_batch_set_p0(dic_batch* db, const double* p0, int mem) {
     return DIC_OK;
 }
 
-int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_cu) {
+int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_subset) {
     CHECK_ARG(db, "dic_batch_set_schedule: NULL batch");
     CHECK_ARG(win_slack >= 0 && win_slack <= 64, "invalid input: win_slack must be in 0..64");
-    CHECK_ARG(waves_per_cu >= 1 && waves_per_cu <= 32, "invalid input: waves_per_cu must be in 1..32");
+    CHECK_ARG(waves_per_subset == 0 || waves_per_subset == 1 || waves_per_subset == 2 || waves_per_subset == 4 ||
+                  waves_per_subset == 8,
+              "invalid input: waves_per_subset must be 0 (automatic), 1, 2, 4 or 8");
     db->b.win_slack = win_slack;
-    db->b.waves_per_cu = waves_per_cu;
+    db->b.waves_per_subset = waves_per_subset;
     return DIC_OK;
 }
 

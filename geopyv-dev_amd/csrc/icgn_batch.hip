@@ -8,44 +8,44 @@ namespace {
 
 using batch::BatchArgs;
 using batch::icgn_batch_kernel;
+using batch::kRedSlots;
 
-template <int M>
-int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds) {
+template <int M, int P, int WPS>
+int launch_mpw(Batch* b, BatchArgs& A, int blocks, size_t lds) {
     Engine* e = b->eng;
-    DIC_HIP_TRY(hipFuncSetAttribute((const void*)icgn_batch_kernel<M>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
-    icgn_batch_kernel<M><<<blocks, 64, lds, e->stream>>>(A);
+    DIC_HIP_TRY(hipFuncSetAttribute((const void*)icgn_batch_kernel<M, P, WPS>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    icgn_batch_kernel<M, P, WPS><<<blocks, 64 * WPS, lds, e->stream>>>(A);
     DIC_HIP_TRY(hipGetLastError());
     return DIC_OK;
+}
+
+// Enough wavefronts per subset to hold the largest subset at 12 pixels per
+// thread (the footprint that stays in registers without spilling).
+template <int M>
+int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds) {
+    const int n = b->max_n;
+    int wps = b->waves_per_subset;
+    if (wps <= 0) wps = n <= 64 * 12 ? 1 : (n <= 128 * 12 ? 2 : (n <= 256 * 12 ? 4 : 8));
+    if (n > 64 * 12 * wps) {
+        set_error("invalid input: subset larger than the wavefronts per subset can hold");
+        return DIC_ERR_INVALID_INPUT;
+    }
+    switch (wps) {
+        case 1: return launch_mpw<M, 12, 1>(b, A, blocks, lds);
+        case 2: return launch_mpw<M, 12, 2>(b, A, blocks, lds);
+        case 4: return launch_mpw<M, 12, 4>(b, A, blocks, lds);
+        case 8: return launch_mpw<M, 12, 8>(b, A, blocks, lds);
+        default: break;
+    }
+    set_error("invalid input: waves per subset must be 1, 2, 4 or 8");
+    return DIC_ERR_INVALID_INPUT;
 }
 
 }  // namespace
 
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
     Engine* e = b->eng;
-    // FP32 slab: 4 floats per pixel, offsets are the template prefix sums
-    const size_t need = (size_t)b->total_px * 4;
-    if (b->slab_floats < need) {
-        if (b->d_slab) dev_free(e, b->d_slab, b->slab_floats * sizeof(float));
-        b->d_slab = nullptr;
-        b->slab_floats = 0;
-        int st = dev_alloc(e, (void**)&b->d_slab, need * sizeof(float));
-        if (st) return st;
-        b->slab_floats = need;
-    }
-    if (!b->slab_off_ready) {
-        std::vector<int64_t> off(b->n);
-        int64_t acc = 0;
-        for (int64_t s = 0; s < b->n; ++s) {
-            off[s] = acc;
-            const int t = b->h_tpl_index[s];
-            acc += b->h_tpl_start[t + 1] - b->h_tpl_start[t];
-        }
-        DIC_HIP_TRY(hipMemcpyAsync(b->d_slab_off, off.data(), sizeof(int64_t) * b->n, hipMemcpyHostToDevice,
-                                   e->stream));
-        DIC_HIP_TRY(hipStreamSynchronize(e->stream));
-        b->slab_off_ready = true;
-    }
     BatchArgs A;
     A.fC = f->coef32;
     A.gC = g->coef32;
@@ -62,8 +62,6 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.tpl_xy = b->d_tpl_xy;
     A.tpl_ext = b->d_tpl_ext;
     A.perm = b->d_order_perm;
-    A.slab = (float4*)b->d_slab;
-    A.slab_off = b->d_slab_off;
     A.p0 = b->d_p0;
     A.p_out = b->d_p;
     A.res = b->d_res;

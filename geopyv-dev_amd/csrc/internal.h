@@ -68,7 +68,7 @@ struct Batch {
     int32_t* d_tpl_ext = nullptr;   // n_tpl x 4: dxmin, dxmax, dymin, dymax
     int32_t max_side = 0;           // largest template box side
     int32_t win_slack = 3;          // deformation slack of the LDS window (pixels per side)
-    int32_t waves_per_cu = 12;      // persistent grid size for the batched engine
+    int32_t waves_per_subset = 0;   // batched engine: wavefronts per subset (0 = smallest that fits)
     int64_t* d_scr_off = nullptr;   // n: offset of each subset's scratch
     double* d_p0 = nullptr;         // n x 12
     double* d_p = nullptr;          // n x 12 final warps

@@ -277,8 +277,8 @@ class Batch:
     def set_p0_device(self, device_ptr):
         check(load().dic_batch_set_p0(self.handle, ctypes.c_void_p(device_ptr), MEM_DEVICE))
 
-    def set_schedule(self, win_slack, waves_per_cu):
-        check(load().dic_batch_set_schedule(self.handle, int(win_slack), int(waves_per_cu)))
+    def set_schedule(self, win_slack, waves_per_subset):
+        check(load().dic_batch_set_schedule(self.handle, int(win_slack), int(waves_per_subset)))
 
     def solve(self, f_img, g_img, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50, tolerance=0.75,
               max_norm=1e-5, history=False):

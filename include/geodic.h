@@ -140,10 +140,11 @@ int dic_batch_history(dic_batch* b, double* out);
  * subset (subset.rs:877-884); available after a DIC_PREC_PLAIN_F64 solve. */
 int dic_batch_subset_arrays(dic_batch* b, int64_t subset, double* f, double* grad_f);
 /* Schedule of the batched engine (DIC_PREC_COMP_F32): LDS window slack in
- * pixels per side beyond the template box, and resident wavefronts per CU
- * of the persistent grid.  The autotuner in the host package picks these;
- * no reference counterpart (the reference has no device schedule). */
-int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_cu);
+ * pixels per side beyond the template box, and wavefronts per subset (1, 2,
+ * 4 or 8; 0 picks the fewest that hold the largest subset at 12 pixels per
+ * thread).  The autotuner in the host package picks these; no reference
+ * counterpart (the reference has no device schedule). */
+int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_subset);
 /* Final warps as a device pointer (n_subsets x 12 doubles) for chaining
  * batches without a host round trip. */
 int dic_batch_device_p(dic_batch* b, const double** out);

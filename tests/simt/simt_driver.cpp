@@ -2,12 +2,13 @@
 // emulation on a synthetic speckle pair and compares every subset with the
 // FP64 oracle.  Built with AddressSanitizer/UBSan by tests/simt/Makefile.
 //
-//   simt_driver ORDER RADIUS NSIDE MAXIT [WIN_SLACK]
+//   simt_driver ORDER RADIUS NSIDE MAXIT [WIN_SLACK] [WPS]
 // exit 0 when every subset matches (|du|,|dv| < 1e-4 px), 1 otherwise.
 #include "simt_hip.h"
 
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 #include "../../geopyv-dev_amd/csrc/icgn_batch_kernel.h"
 
@@ -40,6 +41,7 @@ int main(int argc, char** argv) {
     const int nside = argc > 3 ? atoi(argv[3]) : 3;
     const int max_it = argc > 4 ? atoi(argv[4]) : 50;
     const int slack = argc > 5 ? atoi(argv[5]) : 3;
+    const int wps = argc > 6 ? atoi(argv[6]) : 1;
     const int H = 96, W = 96, B = 20;
     const int HP = H + 2 * B, WP = W + 2 * B;
     uint64_t st = 12345;
@@ -95,9 +97,7 @@ int main(int argc, char** argv) {
         }
     std::vector<int32_t> tpl_index(n, 0), perm(n), orders(n, order);
     for (int s = 0; s < n; ++s) perm[s] = s;
-    std::vector<int64_t> tpl_start = {0, npx}, slab_off(n);
-    for (int s = 0; s < n; ++s) slab_off[s] = (int64_t)s * npx;
-    std::vector<float4> slab((size_t)n * npx);
+    std::vector<int64_t> tpl_start = {0, npx};
     std::vector<dic_subset_result> res(n);
     const int M = order == 0 ? 2 : (order == 1 ? 6 : 12);
     std::vector<double> hist((size_t)n * max_it * 4 + 4, 0.0);
@@ -118,8 +118,6 @@ int main(int argc, char** argv) {
     A.tpl_xy = packed.data();
     A.tpl_ext = ext;
     A.perm = perm.data();
-    A.slab = slab.data();
-    A.slab_off = slab_off.data();
     A.p0 = p0.data();
     A.p_out = pout.data();
     A.res = res.data();
@@ -131,12 +129,22 @@ int main(int argc, char** argv) {
     const int wside = (ext[1] - ext[0] + 1) + 6 + 2 * slack;
     A.win_cap = wside * wside;
     const size_t lds = sizeof(float) * A.win_cap;
+    // P = 12 pixels per thread: one wavefront holds 768 pixels
+    auto run = [&](auto mtag) {
+        constexpr int MM = decltype(mtag)::value;
+        if (wps == 1)
+            simt::launch(n, 64, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 1>(A); });
+        else if (wps == 2)
+            simt::launch(n, 128, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 2>(A); });
+        else
+            simt::launch(n, 256, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 4>(A); });
+    };
     if (M == 2)
-        simt::launch(n, lds, [&] { geodic::batch::icgn_batch_kernel<2>(A); });
+        run(std::integral_constant<int, 2>{});
     else if (M == 6)
-        simt::launch(n, lds, [&] { geodic::batch::icgn_batch_kernel<6>(A); });
+        run(std::integral_constant<int, 6>{});
     else
-        simt::launch(n, lds, [&] { geodic::batch::icgn_batch_kernel<12>(A); });
+        run(std::integral_constant<int, 12>{});
 
     std::vector<double> xy(2 * npx);
     for (int i = 0; i < 2 * npx; ++i) xy[i] = tc[i];

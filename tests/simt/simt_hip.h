@@ -23,7 +23,7 @@
 #define __forceinline__ inline
 #define __shared__ static
 #define __launch_bounds__(...)
-#define GEODIC_BATCH_ATTRS
+#define GEODIC_BATCH_ATTRS(W, OCC)
 #define GEODIC_PLAIN_ATTRS
 #define GEODIC_DYN_LDS(T, name) T* name = (T*)::simt::g_dyn_lds
 
@@ -41,9 +41,10 @@ inline double2 make_double2(double x, double y) { return {x, y}; }
 
 namespace simt {
 constexpr int kLanes = 64;
+constexpr int kMaxThreads = 1024;
 inline std::barrier<>* g_bar = nullptr;
 inline void* g_dyn_lds = nullptr;
-inline uint64_t g_xchg[kLanes];
+inline uint64_t g_xchg[kMaxThreads];
 struct Idx {
     int x = 0, y = 0, z = 0;
 };
@@ -52,6 +53,8 @@ inline Idx g_block, g_blockdim{kLanes, 1, 1}, g_griddim{1, 1, 1};
 
 inline void sync() { g_bar->arrive_and_wait(); }
 
+// Cross-lane read within the calling thread's wavefront (every thread of
+// the block must reach it: it is a block-wide barrier here).
 template <class T>
 T xchg(T v, int src) {
     static_assert(sizeof(T) <= 8, "exchange slot");
@@ -59,24 +62,25 @@ T xchg(T v, int src) {
     std::memcpy(&bits, &v, sizeof(T));
     g_xchg[t_thread.x] = bits;
     sync();
-    uint64_t o = g_xchg[src & (kLanes - 1)];
+    uint64_t o = g_xchg[(t_thread.x & ~(kLanes - 1)) | (src & (kLanes - 1))];
     sync();
     T r;
     std::memcpy(&r, &o, sizeof(T));
     return r;
 }
 
-// Runs `body` for blocks 0..grid-1, one block (64 lane threads) at a time.
-inline void launch(int grid, size_t dyn_lds_bytes, const std::function<void()>& body) {
+// Runs `body` for blocks 0..grid-1, one block (`threads` lanes) at a time.
+inline void launch(int grid, int threads, size_t dyn_lds_bytes, const std::function<void()>& body) {
     std::vector<unsigned char> lds(dyn_lds_bytes + 64, 0xCD);
     g_dyn_lds = lds.data();
     g_griddim.x = grid;
+    g_blockdim.x = threads;
     for (int b = 0; b < grid; ++b) {
         g_block.x = b;
-        std::barrier<> bar(kLanes);
+        std::barrier<> bar(threads);
         g_bar = &bar;
         std::vector<std::thread> th;
-        for (int l = 0; l < kLanes; ++l)
+        for (int l = 0; l < threads; ++l)
             th.emplace_back([l, &body] {
                 t_thread.x = l;
                 body();
@@ -109,7 +113,7 @@ inline int __builtin_amdgcn_readlane(int v, int lane) { return simt::xchg(v, lan
 // whose source is outside its row reads 0 with bound_ctrl, else keeps old.
 inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
     (void)bank_mask;
-    const int lane = simt::t_thread.x;
+    const int lane = simt::t_thread.x & 63;
     const int row = lane >> 4, rl = lane & 15;
     int from = -1;
     if (ctrl < 0x100) {
@@ -127,7 +131,7 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
     } else {
         std::abort();
     }
-    const int v = simt::xchg(src, from < 0 ? lane : from);
+    const int v = simt::xchg(src, from < 0 ? lane : from);  // xchg stays within the wavefront
     if (!((row_mask >> row) & 1)) return old;
     if (from < 0) return bound_ctrl ? 0 : old;
     return v;
