@@ -34,6 +34,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per launch of the batched kernel on this workload, as the
+    FETCH_SIZE / WRITE_SIZE counter summary under profiles/ records them
+    (scaled by a copy of known size); None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "icgn_batch_kernel_uniform_mesh_pmc.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("hbm_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +56,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=40000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="repeat the CPU baseline's frame until this much wall time has passed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--waves-per-subset", type=int, default=0, help="batched-engine schedule (0 = automatic)")
     ap.add_argument("--win-slack", type=int, default=3, help="batched-engine LDS window slack in pixels")
@@ -143,6 +160,7 @@ def main():
     bytes_per_subset = 4 * wx * wy * (1 + it_mean) + 24 * m + 64
     sol_avg = float(np.mean(sol_ms))
     achieved = n * bytes_per_subset / (sol_avg * 1e-3) / 1e9
+    traffic = measured_traffic()
 
     out = {
         "metric": "ICGN subsets/sec (FP32-comp vs FP64 plain, <=1e-4 px); p50/p99 frame latency",
@@ -171,7 +189,7 @@ def main():
         "iterations_mean": round(it_mean, 3),
         "converged_fraction": round(float(np.mean(res32["converged"])), 5),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "bytes_per_subset": round(bytes_per_subset, 1), "kernel": "icgn_batch_kernel<6>"},
         "error_vs_truth_px": {"f32_converged": float(np.abs(res32["p"][:, :2] - truth[:, :2])[res32["converged"] == 1].max()),
                               "fp64_plain_converged": float(np.abs(res64["p"][:, :2] - truth[:, :2])[res64["converged"] == 1].max()),
@@ -187,14 +205,19 @@ def main():
         Cf = O.get_c(ref.astype(np.float64), border, threads)
         Cg = O.get_c(tar.astype(np.float64), border, threads)
         t = O.wall()
-        orc = O.batch_solve(Cf, Cg, ref.shape, border, coords[idx], np.zeros(len(idx), np.int32),
-                            mask.coords, np.array([0, len(xy)]), np.full(len(idx), 1, np.int32), p0[idx],
-                            max_norm=1e-5, max_iterations=50, threads=threads)
-        cpu_s = O.wall() - t
-        out["cpu_baseline"] = {"value": round(len(idx) / cpu_s, 2), "unit": "subsets/s", "cores": threads,
+        reps = 0
+        while True:
+            orc = O.batch_solve(Cf, Cg, ref.shape, border, coords[idx], np.zeros(len(idx), np.int32),
+                                mask.coords, np.array([0, len(xy)]), np.full(len(idx), 1, np.int32), p0[idx],
+                                max_norm=1e-5, max_iterations=50, threads=threads)
+            reps += 1
+            cpu_s = O.wall() - t
+            if cpu_s >= args.cpu_seconds:
+                break
+        out["cpu_baseline"] = {"value": round(reps * len(idx) / cpu_s, 2), "unit": "subsets/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"{len(idx)} of the {n} subsets of one frame (evenly strided), FP64 "
-                                         f"oracle with OpenMP over subsets, {cpu_s:.1f} s; prefilter not timed"}
+                               "sample": f"{reps} x {len(idx)} of the {n} subsets of one frame (evenly strided), "
+                                         f"FP64 oracle with OpenMP over subsets, {cpu_s:.1f} s; prefilter not timed"}
         d32 = res32[idx]
         d64 = res64[idx]
         out["error_vs_oracle"] = {
