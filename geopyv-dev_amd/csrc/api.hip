@@ -153,9 +153,17 @@ int dic_image_create(dic_engine* d, const void* pixels, int pix_type, int mem, i
     im->border = border;
     im->hp = h + 2 * border;
     im->wp = w + 2 * border;
+    im->wp32 = (im->wp + 3) & ~(int64_t)3;
     const size_t np = (size_t)im->hp * im->wp;
+    const size_t np32 = (size_t)im->hp * im->wp32;
     int st = dev_alloc(e, (void**)&im->coef64, np * sizeof(double));
-    if (!st) st = dev_alloc(e, (void**)&im->coef32, np * sizeof(float));
+    if (!st) st = dev_alloc(e, (void**)&im->coef32, np32 * sizeof(float));
+    // the pitch padding of the FP32 copy is never written by the prefilter;
+    // window loads may read it, so give it a defined value
+    if (!st && im->wp32 != im->wp && hipMemsetAsync(im->coef32, 0, np32 * sizeof(float), e->stream) != hipSuccess) {
+        set_error("dic_image_create: clearing the FP32 coefficients failed");
+        st = DIC_ERR_DEVICE;
+    }
     if (!st) st = prefilter_run(e, im, pixels, pix_type, mem);
     if (!st && mem == DIC_MEM_HOST && hipStreamSynchronize(e->stream) != hipSuccess) {
         set_error("dic_image_create: synchronisation failed");
@@ -163,7 +171,7 @@ int dic_image_create(dic_engine* d, const void* pixels, int pix_type, int mem, i
     }
     if (st) {
         dev_free(e, im->coef64, np * sizeof(double));
-        dev_free(e, im->coef32, np * sizeof(float));
+        dev_free(e, im->coef32, np32 * sizeof(float));
         delete di;
         return st;
     }
@@ -211,7 +219,7 @@ void dic_image_destroy(dic_image* di) {
     (void)hipStreamSynchronize(e->stream);
     const size_t np = (size_t)im->hp * im->wp;
     dev_free(e, im->coef64, np * sizeof(double));
-    dev_free(e, im->coef32, np * sizeof(float));
+    dev_free(e, im->coef32, (size_t)im->hp * im->wp32 * sizeof(float));
     delete di;
 }
 

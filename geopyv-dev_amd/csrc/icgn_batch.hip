@@ -45,11 +45,10 @@ int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds) {
 }  // namespace
 
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
-    Engine* e = b->eng;
     BatchArgs A;
     A.fC = f->coef32;
     A.gC = g->coef32;
-    A.wp = f->wp;
+    A.pitch = f->wp32;
     A.border = f->border;
     A.w = (int)f->w;
     A.h = (int)f->h;
@@ -70,10 +69,13 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.max_it = prm->max_iterations;
     A.tol = prm->tolerance;
     A.max_norm = prm->max_norm;
-    // LDS window: the template box plus a few pixels of deformation slack
-    const int side = b->max_side + 6 + 2 * b->win_slack;
-    A.win_cap = side * side;
-    const size_t lds = sizeof(float) * (size_t)A.win_cap;
+    // LDS: the coefficient window (template box, block taps, 16-byte
+    // rounding and the restaging slack, with room for moderate strain)
+    // followed by the template offsets as FP32 pairs
+    const int side = b->max_side + 2 * b->win_slack + 12;
+    A.win_cap = ((side + 3) & ~3) * side;
+    A.win_slack = b->win_slack;
+    const size_t lds = sizeof(float) * (size_t)A.win_cap + 2 * sizeof(float) * (size_t)b->max_n;
     const int blocks = (int)b->n;
     const int order = b->max_order;
     if (order == 0) return launch_m<2>(b, A, blocks, lds);

@@ -4,45 +4,55 @@
 //  * one workgroup of WPS wavefronts per subset, dispatched largest first
 //    so uneven subsets balance across the CUs; thread t owns pixels
 //    t, t + 64·WPS, ... and keeps their reference state in VGPRs for the
-//    whole solve: f - f_m and ∇f in FP32 (P pixels per thread, walked by a
-//    rolled loop whose uniform index the compiler turns into register
-//    indexing, so nothing spills and nothing streams from HBM);
-//  * every iteration stages the coefficient window the warped subset
-//    touches (bounded analytically from the warp and the template extents)
-//    from the FP32 copy of C into LDS with row-coalesced loads, and every
-//    pixel gathers its 6x6 block from LDS; a window that does not fit the
-//    LDS budget falls back to gathering from HBM for that iteration;
+//    whole solve: a = f - f_m and ∇f in FP32 (P pixels per thread, walked
+//    by a rolled loop whose uniform index the compiler turns into register
+//    indexing); the template offsets sit in LDS as FP32 pairs;
+//  * the coefficient window the warped subset touches (bounded analytically
+//    from the warp and the template extents, plus a few pixels of slack) is
+//    staged from the FP32 copy of C into LDS with 16-byte loads, and kept
+//    until an iteration's warp leaves it; every pixel gathers its 6x6 block
+//    from LDS.  When the whole warped subset lies inside the image and the
+//    window, the pixel loop runs without clamps or bounds tests (the fast
+//    path); otherwise every position is clamped as image.rs:130-132 does and
+//    blocks outside the window are read from HBM;
 //  * positions are an integer base (the floor of the subset centre) plus an
-//    FP32 offset relative to it, so the FP32 basis never sees an absolute
-//    coordinate; interpolation and residuals are FP32;
+//    FP32 offset, so the FP32 basis never sees an absolute coordinate; at
+//    setup every pixel shares the centre's fraction, so the basis is
+//    computed once per subset;
 //  * each iteration is ONE pass over the pixels.  With g' = g - f_m (a
 //    shift by the reference mean keeps the sums well conditioned) the pass
-//    accumulates, in FP64,
-//        s1 = Σ g',  s2 = Σ g'²,  s3 = Σ a·g',  B_j = Σ sdi_j·g'   (a = f - f_m)
+//    accumulates
+//        s1 = Σ g',  s2 = Σ g'²,  s3 = Σ a·g',  B_j = Σ sdi_j·g'
 //    and the reference's per-pixel forms (subset.rs:1162-1176, 418-446)
 //    follow exactly, summed in another order:
 //        g_m = f_m + s1/n,   Δg² = s2 - s1²/n,
 //        ∇ZNSSD_j = Σ sdi_j (a - (Δf/Δg)(g - g_m)) = A_j - (Δf/Δg)(B_j - (s1/n) S_j),
 //        ZNSSD = 2 - 2 (s3 - (s1/n) Σa) / (Δf Δg),
 //    with A_j = Σ sdi_j a and S_j = Σ sdi_j fixed at setup;
-//  * every sum accumulates in FP64: on CDNA4 v_fma_f64 issues at the rate of
-//    an unpacked FP32 FMA, so FP64 accumulation costs no more than a
-//    compensated FP32 two-sum and is at least as accurate.  Wavefront sums
-//    are DPP trees, cross-wave sums go through LDS in fixed wave order;
-//  * the Hessian is factorised once (FP64 Cholesky) and each update
-//    Δp = -(LLᵀ)⁻¹∇ZNSSD and the composition run in FP64;
-//  * the final ZNSSD and warp-adequacy diagnostic re-evaluate g once at the
-//    warp of the last iteration (the reference's final g buffer).
+//  * per-pixel arithmetic and each thread's partial sums (at most P terms)
+//    are FP32; the partials are widened to FP64 for the wavefront DPP trees
+//    and the cross-wavefront sums (LDS, fixed wave order), so a sum's error
+//    is that of a dozen FP32 additions;
+//  * the Hessian is factorised once (FP64 Cholesky, reciprocal diagonal);
+//    each update Δp = -(LLᵀ)⁻¹∇ZNSSD and the composition run in FP64;
+//  * the warp-adequacy diagnostic (subset.rs:748-820) needs only reference
+//    data except for the final residual e: its gradient standardisation and
+//    mode statistics are computed at setup, and the final pass that
+//    re-evaluates g at the last iteration's warp accumulates Σe², Σe and the
+//    projections Σ sc_q·e together (Σ sc_q·(e - ē) = Σ sc_q·e - ē Σ sc_q).
 #pragma once
 
 #include "bspline.h"
 #include "linalg.h"
 #include "../../include/geodic.h"
 
-// Resident wavefronts per SIMD the register budget must allow: 4 (128
-// VGPRs) for orders 0 and 1, 2 for the 12-parameter warp.
 #ifndef GEODIC_BATCH_ATTRS
 #define GEODIC_BATCH_ATTRS(WPS, OCC) __launch_bounds__(64 * (WPS)) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+#endif
+// Resident wavefronts per SIMD the register budget is sized for (orders 0
+// and 1; the 12-parameter warp takes what it needs).
+#ifndef GEODIC_BATCH_OCC
+#define GEODIC_BATCH_OCC 3
 #endif
 #ifndef GEODIC_DYN_LDS
 #define GEODIC_DYN_LDS(T, name) extern __shared__ T name[]
@@ -54,7 +64,7 @@ namespace batch {
 struct BatchArgs {
     const float* fC;
     const float* gC;
-    int64_t wp;
+    int64_t pitch;  // row pitch of the FP32 coefficient arrays, a multiple of 4
     int border;
     int w, h;
     double wmax, hmax;
@@ -73,7 +83,8 @@ struct BatchArgs {
     int max_it;
     double tol;
     double max_norm;
-    int win_cap;  // floats of LDS for the coefficient window
+    int win_cap;   // floats of LDS for the coefficient window (a multiple of 4)
+    int win_slack; // pixels of slack around a restaged window
 };
 
 constexpr int kRedSlots = 40;  // doubles per wavefront in the cross-wave reduction buffer
@@ -118,59 +129,102 @@ __device__ __forceinline__ void warp_box(const double* p, const int32_t e[4], do
     }
 }
 
+// Staged coefficient window: rows [y0, y0 + wh), columns [x0, x0 + ww) of
+// C, row stride ww floats in LDS (x0 and ww multiples of 4).  Pixels whose
+// cell lies in [cx_lo, cx_hi] x [cy_lo, cy_hi] find their whole 6x6 block
+// inside.  lds == false: nothing staged, every gather reads HBM.
 struct Window {
-    int x0, y0, ww, wh;  // origin in coefficient space and size
+    int x0, y0, ww, wh;
+    int cx_lo, cx_hi, cy_lo, cy_hi;
     bool lds;
 };
 
-// Window of absolute positions [ax0, ax1] x [ay0, ay1] after the clamp.
-__device__ __forceinline__ Window make_window(double ax0, double ax1, double ay0, double ay1, const BatchArgs& A) {
-    // margin covers the FP32 rounding of per-pixel positions
-    const double e = 1e-3;
-    ax0 = clamp_ref(ax0 - e, 0.0, A.wmax);
-    ax1 = clamp_ref(ax1 + e, 0.0, A.wmax);
-    ay0 = clamp_ref(ay0 - e, 0.0, A.hmax);
-    ay1 = clamp_ref(ay1 + e, 0.0, A.hmax);
+__device__ __forceinline__ Window no_window() {
     Window W;
-    // a NaN bound (diverged warp) gives an empty window: every gather then
-    // goes to HBM through the bounds-safe index path of eval32
-    if (!(ax0 <= ax1) || !(ay0 <= ay1)) {
-        W.x0 = W.y0 = W.ww = W.wh = 0;
-        W.lds = false;
-        return W;
-    }
-    const int ix0 = (int)floor(ax0), ix1 = (int)floor(ax1);
-    const int iy0 = (int)floor(ay0), iy1 = (int)floor(ay1);
-    W.x0 = uni(ix0 + A.border - 2);
-    W.y0 = uni(iy0 + A.border - 2);
-    W.ww = uni(ix1 - ix0 + 6);
-    W.wh = uni(iy1 - iy0 + 6);
-    W.lds = (W.ww * W.wh) <= A.win_cap;
+    W.x0 = W.y0 = W.ww = W.wh = 0;
+    W.cx_lo = W.cy_lo = 1;
+    W.cx_hi = W.cy_hi = 0;
+    W.lds = false;
     return W;
 }
 
-// Stage C[y0:y0+wh, x0:x0+ww] into LDS: thread t takes column t % 64 of
-// row block t / 64 (so each row's loads coalesce) and walks rows with a
-// pointer increment; eight rows in flight per group.
+// Cells touched by the absolute box [ax0, ax1] x [ay0, ay1] after the clamp
+// (image.rs:130-131); false for a NaN box.
+__device__ __forceinline__ bool box_cells(double ax0, double ax1, double ay0, double ay1, const BatchArgs& A,
+                                          int c[4]) {
+    ax0 = clamp_ref(ax0, 0.0, A.wmax);
+    ax1 = clamp_ref(ax1, 0.0, A.wmax);
+    ay0 = clamp_ref(ay0, 0.0, A.hmax);
+    ay1 = clamp_ref(ay1, 0.0, A.hmax);
+    if (!(ax0 <= ax1) || !(ay0 <= ay1)) return false;
+    c[0] = uni((int)floor(ax0));
+    c[1] = uni((int)floor(ax1));
+    c[2] = uni((int)floor(ay0));
+    c[3] = uni((int)floor(ay1));
+    return true;
+}
+
+// Window holding the blocks of cells [c0, c1] x [c2, c3] (all in the image).
+// Columns round out to 16-byte boundaries; the FP32 rows are pitched to
+// 16 bytes, so x0 + ww never passes the pitch.
+__device__ __forceinline__ Window window_for(const int c[4], const BatchArgs& A) {
+    Window W;
+    W.x0 = (c[0] + A.border - 2) & ~3;
+    const int x1 = c[1] + A.border + 3;
+    W.ww = (x1 - W.x0 + 4) & ~3;
+    W.y0 = c[2] + A.border - 2;
+    W.wh = c[3] - c[2] + 6;
+    W.cx_lo = W.x0 - A.border + 2;
+    W.cx_hi = W.x0 + W.ww - A.border - 4;
+    W.cy_lo = c[2];
+    W.cy_hi = c[3];
+    W.lds = W.ww * W.wh <= A.win_cap;
+    return W;
+}
+
+__device__ __forceinline__ bool covers(const Window& W, const int c[4]) {
+    return W.lds && c[0] >= W.cx_lo && c[1] <= W.cx_hi && c[2] >= W.cy_lo && c[3] <= W.cy_hi;
+}
+
+// Stage the window into LDS: thread t copies 16-byte chunks t, t + NT, ...
+// of the row-major chunk grid, four loads in flight.
 template <int NT>
-__device__ __forceinline__ void stage_window(float* win, const float* __restrict__ C, int64_t wp, const Window& W,
+__device__ __forceinline__ void stage_window(float* win, const float* __restrict__ C, int64_t pitch, const Window& W,
                                              int tid) {
-    constexpr int RB = NT / 64;  // rows handled concurrently by the workgroup
-    const int c0 = tid & 63, rb = tid >> 6;
-    const float* __restrict__ src0 = C + (int64_t)W.y0 * wp + W.x0;
-    const int wpi = (int)wp;  // row offsets within a window fit in 32 bits
-    for (int cb = 0; cb < W.ww; cb += 64) {
-        const int c = cb + c0;
-        if (c < W.ww) {
-            int r = rb;
-            for (; r + 7 * RB < W.wh; r += 8 * RB) {
-                float v[8];
+    const int cpr = W.ww >> 2;  // chunks per row
+    const int total = cpr * W.wh;
+    const float* __restrict__ src0 = C + (int64_t)W.y0 * pitch + W.x0;
+    const int pi = (int)pitch;  // offsets inside a window fit in 32 bits
+    int r = tid / cpr;
+    int c = tid - r * cpr;
+    const int dr = NT / cpr, dc = NT - (NT / cpr) * cpr;
+    int k = tid;
+    for (; k + 3 * NT < total; k += 4 * NT) {
+        int rr[4], cc[4];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) v[k] = src0[(r + k * RB) * wpi + c];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) win[(r + k * RB) * W.ww + c] = v[k];
+        for (int j = 0; j < 4; ++j) {
+            rr[j] = r;
+            cc[j] = c;
+            r += dr;
+            c += dc;
+            if (c >= cpr) {
+                c -= cpr;
+                ++r;
             }
-            for (; r < W.wh; r += RB) win[r * W.ww + c] = src0[r * wpi + c];
+        }
+        float4 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = *(const float4*)(src0 + rr[j] * pi + 4 * cc[j]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) *(float4*)(win + rr[j] * W.ww + 4 * cc[j]) = v[j];
+    }
+    for (; k < total; k += NT) {
+        *(float4*)(win + r * W.ww + 4 * c) = *(const float4*)(src0 + r * pi + 4 * c);
+        r += dr;
+        c += dc;
+        if (c >= cpr) {
+            c -= cpr;
+            ++r;
         }
     }
 }
@@ -193,64 +247,111 @@ __device__ __forceinline__ void cell_frac(int base, float local, int n, int& cel
     }
 }
 
-// FP32 value (and optionally gradient) at cell (cx, cy) + (tx, ty).
-template <bool GRAD>
-__device__ __forceinline__ float eval32(const float* win, const float* __restrict__ C, int64_t wp, const Window& W,
-                                        int border, int cellx, int celly, float tx, float ty, float& gx, float& gy) {
-    float a[6], b[6];
-    basis_f32(ty, a);
-    basis_f32(tx, b);
-    const int lx = cellx + border - 2 - W.x0, ly = celly + border - 2 - W.y0;
-    const bool in = W.lds && lx >= 0 && ly >= 0 && lx <= W.ww - 6 && ly <= W.wh - 6;
-    float val = 0.f, ggx = 0.f, ggy = 0.f;
-    float da[6], db[6];
-    if (GRAD) {
-        dbasis_f32(ty, da);
-        dbasis_f32(tx, db);
+// 6x6 contraction Σ_r a_r Σ_k blk[r][k] b_k, row stride `stride`.
+__device__ __forceinline__ float contract6(const float* blk, int stride, const float a[6], const float b[6]) {
+    float val = 0.f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        const float* row = blk + r * stride;
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s = fmaf(row[k], b[k], s);
+        val = fmaf(a[r], s, val);
     }
-    // Two explicit gather paths so each compiles to its own address space
-    // (ds_read from the staged window, global_load otherwise).
-    if (in) {
-        const float* base = win + ly * W.ww + lx;
+    return val;
+}
+
+// The 6x6 block into registers, every read issued before any use (the
+// scheduling barrier keeps the basis arithmetic that follows from being
+// interleaved with the reads, so the 18 LDS round trips overlap).
+__device__ __forceinline__ void gather6(const float* blk, int stride, float c[36]) {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            const float* row = base + r * W.ww;
-            float s = 0.f, t = 0.f;
+    for (int r = 0; r < 6; ++r)
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                s = fmaf(row[k], b[k], s);
-                if (GRAD) t = fmaf(row[k], db[k], t);
-            }
-            val = fmaf(a[r], s, val);
-            if (GRAD) {
-                ggx = fmaf(a[r], t, ggx);
-                ggy = fmaf(da[r], s, ggy);
-            }
+        for (int k = 0; k < 6; ++k) c[6 * r + k] = blk[r * stride + k];
+    GEODIC_SCHED_BARRIER();
+}
+
+__device__ __forceinline__ float contract6r(const float c[36], const float a[6], const float b[6]) {
+    float val = 0.f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        float s = c[6 * r] * b[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) s = fmaf(c[6 * r + k], b[k], s);
+        val = (r == 0) ? a[0] * s : fmaf(a[r], s, val);
+    }
+    return val;
+}
+
+__device__ __forceinline__ float contract6r_grad(const float c[36], const float a[6], const float b[6],
+                                                 const float da[6], const float db[6], float& gx, float& gy) {
+    float val = 0.f, ggx = 0.f, ggy = 0.f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        float s = c[6 * r] * b[0], t = c[6 * r] * db[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) {
+            s = fmaf(c[6 * r + k], b[k], s);
+            t = fmaf(c[6 * r + k], db[k], t);
         }
-    } else {
-        const float* __restrict__ base = C + (int64_t)(celly + border - 2) * wp + (cellx + border - 2);
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            const float* row = base + r * wp;
-            float s = 0.f, t = 0.f;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                s = fmaf(row[k], b[k], s);
-                if (GRAD) t = fmaf(row[k], db[k], t);
-            }
-            val = fmaf(a[r], s, val);
-            if (GRAD) {
-                ggx = fmaf(a[r], t, ggx);
-                ggy = fmaf(da[r], s, ggy);
-            }
-        }
+        val = fmaf(a[r], s, val);
+        ggx = fmaf(a[r], t, ggx);
+        ggy = fmaf(da[r], s, ggy);
     }
     gx = ggx;
     gy = ggy;
     return val;
 }
 
-// Sums V values over the workgroup: DPP tree per wavefront, then the
+// Value, ∂x and ∂y from one 6x6 block: ∂x with the derivative basis in x,
+// ∂y with it in y (image.rs:155-178).
+__device__ __forceinline__ float contract6_grad(const float* blk, int stride, const float a[6], const float b[6],
+                                                const float da[6], const float db[6], float& gx, float& gy) {
+    float val = 0.f, ggx = 0.f, ggy = 0.f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        const float* row = blk + r * stride;
+        float s = 0.f, t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            s = fmaf(row[k], b[k], s);
+            t = fmaf(row[k], db[k], t);
+        }
+        val = fmaf(a[r], s, val);
+        ggx = fmaf(a[r], t, ggx);
+        ggy = fmaf(da[r], s, ggy);
+    }
+    gx = ggx;
+    gy = ggy;
+    return val;
+}
+
+// General gather at cell (cx, cy) + (tx, ty): LDS when the block is inside
+// the window, HBM otherwise (two explicit paths so each compiles to its own
+// address space).
+template <bool GRAD>
+__device__ __forceinline__ float eval32(const float* win, const float* __restrict__ C, int64_t pitch, const Window& W,
+                                        int border, int cellx, int celly, float tx, float ty, float& gx, float& gy) {
+    float a[6], b[6], da[6], db[6];
+    basis_f32(ty, a);
+    basis_f32(tx, b);
+    if (GRAD) {
+        dbasis_f32(ty, da);
+        dbasis_f32(tx, db);
+    }
+    const bool in = W.lds && cellx >= W.cx_lo && cellx <= W.cx_hi && celly >= W.cy_lo && celly <= W.cy_hi;
+    if (in) {
+        const float* blk = win + (celly - W.cy_lo) * W.ww + (cellx - W.cx_lo);
+        if (GRAD) return contract6_grad(blk, W.ww, a, b, da, db, gx, gy);
+        return contract6(blk, W.ww, a, b);
+    }
+    const float* __restrict__ blk = C + (int64_t)(celly + border - 2) * pitch + (cellx + border - 2);
+    if (GRAD) return contract6_grad(blk, (int)pitch, a, b, da, db, gx, gy);
+    return contract6(blk, (int)pitch, a, b);
+}
+
+// Sums V FP64 values over the workgroup: DPP tree per wavefront, then the
 // wavefront partials in wave order through LDS.  Result uniform (SGPRs).
 template <int WPS, int V>
 __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int wave) {
@@ -274,8 +375,17 @@ __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int
     }
 }
 
+// FP32 per-thread partials widened to FP64 and summed over the workgroup.
+template <int WPS, int V>
+__device__ __forceinline__ void block_sum32(const float* f, double* v, double* sRed, int lane, int wave) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[j] = (double)f[j];
+    block_sum<WPS, V>(v, sRed, lane, wave);
+}
+
+// steepest-descent row of one pixel, FP32 (subset.rs:311-333)
 template <int M>
-__device__ __forceinline__ void sdi64(double gx, double gy, double dx, double dy, double* s) {
+__device__ __forceinline__ void sdi32(float gx, float gy, float dx, float dy, float* s) {
     s[0] = gx;
     s[1] = gy;
     if (M >= 6) {
@@ -285,7 +395,7 @@ __device__ __forceinline__ void sdi64(double gx, double gy, double dx, double dy
         s[5] = gy * dy;
     }
     if (M == 12) {
-        const double hx = 0.5 * dx * dx, hy = 0.5 * dy * dy, xy = dx * dy;
+        const float hx = 0.5f * dx * dx, hy = 0.5f * dy * dy, xy = dx * dy;
         s[6] = gx * hx;
         s[7] = gy * hx;
         s[8] = gx * xy;
@@ -297,78 +407,176 @@ __device__ __forceinline__ void sdi64(double gx, double gy, double dx, double dy
 
 // Hessian rows [A0, A1) from the register-resident gradients.
 template <int M, int P, int WPS, int A0, int A1>
-__device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr, const int32_t* __restrict__ offs,
-                                             int n, int tid, int lane, int wave, double* sH, double* sRed) {
+__device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr, const float2* sXY, int n, int tid,
+                                             int lane, int wave, double* sH, double* sRed) {
     constexpr int NT = 64 * WPS;
     constexpr int NE = (A1 - A0) * M - ((A1 - 1) * A1 - (A0 - 1) * A0) / 2;
-    double acc[NE > 0 ? NE : 1];
+    float acc[NE > 0 ? NE : 1];
 #pragma unroll
-    for (int q = 0; q < NE; ++q) acc[q] = 0.0;
+    for (int q = 0; q < NE; ++q) acc[q] = 0.f;
 #pragma unroll 1
     for (int k = 0; k < P; ++k) {
         const int i = tid + NT * k;
         if (i < n) {
-            const int32_t o = offs[i];
-            double sd[M];
-            sdi64<M>((double)gxr[k], (double)gyr[k], (double)unpack_dx(o), (double)unpack_dy(o), sd);
+            const float2 d = sXY[i];
+            float sd[M];
+            sdi32<M>(gxr[k], gyr[k], d.x, d.y, sd);
             int q = 0;
 #pragma unroll
             for (int a = A0; a < A1; ++a)
 #pragma unroll
                 for (int b = a; b < M; ++b) {
-                    acc[q] = fma(sd[a], sd[b], acc[q]);
+                    acc[q] = fmaf(sd[a], sd[b], acc[q]);
                     ++q;
                 }
         }
     }
-    block_sum<WPS, NE>(acc, sRed, lane, wave);
+    double v[NE > 0 ? NE : 1];
+    block_sum32<WPS, NE>(acc, v, sRed, lane, wave);
     if (tid == 0) {
         int q = 0;
 #pragma unroll
         for (int a = A0; a < A1; ++a)
 #pragma unroll
             for (int b = a; b < M; ++b) {
-                sH[a * M + b] = acc[q];
-                sH[b * M + a] = acc[q];
+                sH[a * M + b] = v[q];
+                sH[b * M + a] = v[q];
                 ++q;
             }
     }
 }
 
-// warped offset of pixel (dx, dy) relative to the integer base, FP32
+// Warped position relative to the integer base, FP32.  q2p = 1 + u_x and
+// q5p = 1 + v_y fold the identity into the gradient terms; h[] holds the
+// second-order coefficients already halved where the warp halves them.
 template <int M>
-__device__ __forceinline__ void warp_local32(const float* q, float fx, float fy, float dx, float dy, float& ox,
-                                             float& oy) {
-    // q[0..1] = u, v; q[2..5] = u_x, v_x, u_y, v_y; q[6..11] second order
-    float wx = q[0], wy = q[1];
+__device__ __forceinline__ void warp_pos(const float* q, float c0x, float c0y, float dx, float dy, float& ox,
+                                         float& oy) {
+    if (M == 2) {
+        ox = c0x + dx;
+        oy = c0y + dy;
+    } else if (M == 6) {
+        ox = fmaf(q[2], dx, fmaf(q[4], dy, c0x));
+        oy = fmaf(q[3], dx, fmaf(q[5], dy, c0y));
+    } else {
+        const float xx = dx * dx, xy = dx * dy, yy = dy * dy;
+        ox = fmaf(q[2], dx, fmaf(q[4], dy, fmaf(q[6], xx, fmaf(q[8], xy, fmaf(q[10], yy, c0x)))));
+        oy = fmaf(q[3], dx, fmaf(q[5], dy, fmaf(q[7], xx, fmaf(q[9], xy, fmaf(q[11], yy, c0y)))));
+    }
+}
+
+// Per-iteration FP32 accumulators: s1, s2, s3, B_0..B_{M-1}.
+template <int M>
+__device__ __forceinline__ void accumulate(float* acc, float gp, float a, float gx, float gy, float dx, float dy) {
+    acc[0] += gp;
+    acc[1] = fmaf(gp, gp, acc[1]);
+    acc[2] = fmaf(a, gp, acc[2]);
+    const float tx = gx * gp, ty = gy * gp;
+    acc[3] += tx;
+    acc[4] += ty;
     if (M >= 6) {
-        wx = fmaf(q[2], dx, fmaf(q[4], dy, wx));
-        wy = fmaf(q[3], dx, fmaf(q[5], dy, wy));
+        acc[5] = fmaf(tx, dx, acc[5]);
+        acc[6] = fmaf(ty, dx, acc[6]);
+        acc[7] = fmaf(tx, dy, acc[7]);
+        acc[8] = fmaf(ty, dy, acc[8]);
     }
     if (M == 12) {
         const float hx = 0.5f * dx * dx, hy = 0.5f * dy * dy, xy = dx * dy;
-        wx = fmaf(q[6], hx, fmaf(q[8], xy, fmaf(q[10], hy, wx)));
-        wy = fmaf(q[7], hx, fmaf(q[9], xy, fmaf(q[11], hy, wy)));
+        acc[9] = fmaf(tx, hx, acc[9]);
+        acc[10] = fmaf(ty, hx, acc[10]);
+        acc[11] = fmaf(tx, xy, acc[11]);
+        acc[12] = fmaf(ty, xy, acc[12]);
+        acc[13] = fmaf(tx, hy, acc[13]);
+        acc[14] = fmaf(ty, hy, acc[14]);
     }
-    ox = fx + dx + wx;
-    oy = fy + dy + wy;
+}
+
+// The omitted-mode monomials of one pixel (subset.rs:760-775).
+template <int M>
+__device__ __forceinline__ void modes32(float xi, float ze, float* mo) {
+    if (M == 2) {
+        mo[0] = xi;
+        mo[1] = ze;
+    } else if (M == 6) {
+        mo[0] = xi * xi;
+        mo[1] = xi * ze;
+        mo[2] = ze * ze;
+    } else {
+        mo[0] = xi * xi * xi;
+        mo[1] = xi * xi * ze;
+        mo[2] = xi * ze * ze;
+        mo[3] = ze * ze * ze;
+    }
+}
+
+// x = -(L Lᵀ)⁻¹ b with the packed lower factor and its reciprocal diagonal.
+template <int M>
+__device__ __forceinline__ void chol_solve_neg_rd(const double* L, const double* rd, const double* b, double* x) {
+    double y[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double s = b[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s = fma(-L[i * (i + 1) / 2 + k], y[k], s);
+        y[i] = s * rd[i];
+    }
+#pragma unroll
+    for (int i = M - 1; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int k = i + 1; k < M; ++k) s = fma(-L[k * (k + 1) / 2 + i], x[k], s);
+        x[i] = s * rd[i];
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = -x[i];
+}
+
+// Order-1 composition W(p)·W(dp)⁻¹ (subset.rs:499-528) with one reciprocal
+// of the determinant; false when W(dp) is singular.
+__device__ __forceinline__ bool compose6_fast(const double* p, const double* dp, double* out) {
+    const double a = 1.0 + dp[2], b = dp[4], c = dp[0];
+    const double d = dp[3], e = 1.0 + dp[5], f = dp[1];
+    const double det = a * e - b * d;
+    if (det == 0.0) return false;
+    const double r = 1.0 / det;
+    // inverse of [[a b c] [d e f] [0 0 1]]
+    const double i00 = e * r, i01 = -b * r, i02 = (b * f - c * e) * r;
+    const double i10 = -d * r, i11 = a * r, i12 = (c * d - a * f) * r;
+    const double o00 = 1.0 + p[2], o01 = p[4], o02 = p[0];
+    const double o10 = p[3], o11 = 1.0 + p[5], o12 = p[1];
+    const double n00 = o00 * i00 + o01 * i10, n01 = o00 * i01 + o01 * i11, n02 = o00 * i02 + o01 * i12 + o02;
+    const double n10 = o10 * i00 + o11 * i10, n11 = o10 * i01 + o11 * i11, n12 = o10 * i02 + o11 * i12 + o12;
+    out[0] = n02;
+    out[1] = n12;
+    out[2] = n00 - 1.0;
+    out[3] = n10;
+    out[4] = n01;
+    out[5] = n11 - 1.0;
+    return true;
 }
 
 template <int M, int P, int WPS>
-__global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : 3)) icgn_batch_kernel(BatchArgs A) {
+__global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_batch_kernel(BatchArgs A) {
     constexpr int NT = 64 * WPS;
     constexpr int NL = M * (M + 1) / 2;
     constexpr int K = Modes<M>::K;
-    GEODIC_DYN_LDS(float, win);
+    constexpr int NA = 3 + M;  // per-iteration sums
+    GEODIC_DYN_LDS(float, lds);
+    float* win = lds;
+    float2* sXY = (float2*)(lds + A.win_cap);
     __shared__ double sH[M * M];
     __shared__ double sL[NL];
-    __shared__ double sScr[144];
+    __shared__ double sRd[M];
+    __shared__ double sAS[2 * M];           // A_j, S_j
+    __shared__ float sDg[4 + 6 * Modes<M>::K];  // mx, my, 1/sdx, 1/sdy, rs, 1/rv, Σsc
+    __shared__ double sScr[M == 12 ? 144 : 1];  // order-2 composition workspace
     __shared__ double sP[12];
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const double qnan = __longlong_as_double(0x7ff8000000000000LL);
+    const double margin = 1e-3;  // covers the FP32 rounding of per-pixel positions
 
     // One workgroup per subset, dispatched largest first (perm).  A lane-0
     // atomic work counter broadcast with a shuffle is avoided on purpose:
@@ -389,7 +597,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : 3)) icgn_batch_kernel(Bat
     const double bxd = floor(xc), byd = floor(yc);
     const bool centre_ok = bxd > -1.0e9 && bxd < 1.0e9 && byd > -1.0e9 && byd < 1.0e9;
     const int bx = centre_ok ? (int)bxd : 0, by = centre_ok ? (int)byd : 0;
-    const float fxc = (float)(xc - bxd), fyc = (float)(yc - byd);
+    const float fxc = uni((float)(xc - bxd)), fyc = uni((float)(yc - byd));
     const double dn = (double)n;
     dic_subset_result* R = A.res + s;
     if (tid == 0) {
@@ -410,69 +618,111 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : 3)) icgn_batch_kernel(Bat
         if (tid == 0) R->status = DIC_ERR_INVALID_INPUT;
         return;
     }
+    for (int i = tid; i < n; i += NT) {
+        const int32_t o = offs[i];
+        sXY[i] = make_float2((float)unpack_dx(o), (float)unpack_dy(o));
+    }
 
     float am[P], gxr[P], gyr[P];
 
     // ---- Subset::new on the staged reference window ----
     double f_m, delta_f, sa;
-    double Aj[M], Sj[M];
     {
-        const Window W = make_window(xc + ext[0], xc + ext[1], yc + ext[2], yc + ext[3], A);
-        if (W.lds) stage_window<NT>(win, A.fC, A.wp, W, tid);
+        const double ax0 = xc + ext[0] - margin, ax1 = xc + ext[1] + margin;
+        const double ay0 = yc + ext[2] - margin, ay1 = yc + ext[3] + margin;
+        int c[4];
+        Window W = no_window();
+        if (box_cells(ax0, ax1, ay0, ay1, A, c)) W = window_for(c, A);
+        if (W.lds) stage_window<NT>(win, A.fC, A.pitch, W, tid);
         __syncthreads();
-        double red[2] = {0.0, 0.0};  // Σf, SSSIG
-#pragma unroll 1
-        for (int k = 0; k < P; ++k) {
-            const int i = tid + NT * k;
-            float v = 0.f, gx = 0.f, gy = 0.f;
-            if (i < n) {
-                const int32_t o = offs[i];
-                int cx, cy;
-                float tx, ty;
-                cell_frac(bx, fxc + (float)unpack_dx(o), A.w, cx, tx);
-                cell_frac(by, fyc + (float)unpack_dy(o), A.h, cy, ty);
-                v = eval32<true>(win, A.fC, A.wp, W, A.border, cx, cy, tx, ty, gx, gy);
-                red[0] += (double)v;
-                red[1] += 0.5 * ((double)gx * gx + (double)gy * gy);
-            }
-            am[k] = v;
-            gxr[k] = gx;
-            gyr[k] = gy;
-        }
-        block_sum<WPS, 2>(red, sRed, lane, wave);
-        f_m = uni(red[0] / dn);
-        const double ss = red[1];
-        double v2[3 + 2 * M];
+        const bool fast = W.lds && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
+        float red[2] = {0.f, 0.f};  // Σf, SSSIG
+        if (fast) {
+            // every pixel sits at an integer offset from the centre: one
+            // basis for the whole subset
+            float ba[6], bb[6], bda[6], bdb[6];
+            basis_f32(fyc, ba);
+            basis_f32(fxc, bb);
+            dbasis_f32(fyc, bda);
+            dbasis_f32(fxc, bdb);
 #pragma unroll
-        for (int j = 0; j < 3 + 2 * M; ++j) v2[j] = 0.0;
+            for (int q = 0; q < 6; ++q) {
+                ba[q] = uni(ba[q]);
+                bb[q] = uni(bb[q]);
+                bda[q] = uni(bda[q]);
+                bdb[q] = uni(bdb[q]);
+            }
+            const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
+            const float wwf = (float)W.ww;
+#pragma unroll 1
+            for (int k = 0; k < P; ++k) {
+                const int i = tid + NT * k;
+                float v = 0.f, gx = 0.f, gy = 0.f;
+                if (i < n) {
+                    const float2 d = sXY[i];
+                    float cc[36];
+                    gather6(win + cb + (int)fmaf(d.y, wwf, d.x), W.ww, cc);
+                    v = contract6r_grad(cc, ba, bb, bda, bdb, gx, gy);
+                    red[0] += v;
+                    red[1] = fmaf(0.5f, fmaf(gx, gx, gy * gy), red[1]);
+                }
+                am[k] = v;
+                gxr[k] = gx;
+                gyr[k] = gy;
+            }
+        } else {
+#pragma unroll 1
+            for (int k = 0; k < P; ++k) {
+                const int i = tid + NT * k;
+                float v = 0.f, gx = 0.f, gy = 0.f;
+                if (i < n) {
+                    const float2 d = sXY[i];
+                    int cx, cy;
+                    float tx, ty;
+                    cell_frac(bx, fxc + d.x, A.w, cx, tx);
+                    cell_frac(by, fyc + d.y, A.h, cy, ty);
+                    v = eval32<true>(win, A.fC, A.pitch, W, A.border, cx, cy, tx, ty, gx, gy);
+                    red[0] += v;
+                    red[1] = fmaf(0.5f, fmaf(gx, gx, gy * gy), red[1]);
+                }
+                am[k] = v;
+                gxr[k] = gx;
+                gyr[k] = gy;
+            }
+        }
+        double rd[2];
+        block_sum32<WPS, 2>(red, rd, sRed, lane, wave);
+        f_m = uni(rd[0] / dn);
+        const double ss = rd[1];
+        float v2[2 + 2 * M];
+#pragma unroll
+        for (int j = 0; j < 2 + 2 * M; ++j) v2[j] = 0.f;
 #pragma unroll 1
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
-                const int32_t o = offs[i];
-                const double a = (double)am[k] - f_m;
-                v2[0] = fma(a, a, v2[0]);
+                const float2 d = sXY[i];
+                const float a = (float)((double)am[k] - f_m);
+                v2[0] = fmaf(a, a, v2[0]);
                 v2[1] += a;
-                double sd[M];
-                sdi64<M>((double)gxr[k], (double)gyr[k], (double)unpack_dx(o), (double)unpack_dy(o), sd);
+                float sd[M];
+                sdi32<M>(gxr[k], gyr[k], d.x, d.y, sd);
 #pragma unroll
                 for (int j = 0; j < M; ++j) {
-                    v2[3 + j] = fma(sd[j], a, v2[3 + j]);
-                    v2[3 + M + j] += sd[j];
+                    v2[2 + j] = fmaf(sd[j], a, v2[2 + j]);
+                    v2[2 + M + j] += sd[j];
                 }
-                am[k] = (float)a;
+                am[k] = a;
             }
         }
-        block_sum<WPS, 3 + 2 * M>(v2, sRed, lane, wave);
-        const double sq = v2[0];
-        sa = v2[1];
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-            Aj[j] = v2[3 + j];
-            Sj[j] = v2[3 + M + j];
-        }
+        double w2[2 + 2 * M];
+        block_sum32<WPS, 2 + 2 * M>(v2, w2, sRed, lane, wave);
+        const double sq = w2[0];
+        sa = w2[1];
         delta_f = uni(sqrt(sq));
         if (tid == 0) {
+#pragma unroll
+            for (int j = 0; j < 2 * M; ++j) sAS[j] = w2[2 + j];
             R->f_m = f_m;
             R->delta_f = delta_f;
             R->sssig = ss;
@@ -484,24 +734,100 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : 3)) icgn_batch_kernel(Bat
         return;
     }
     if constexpr (M == 12) {
-        hessian_rows<M, P, WPS, 0, 3>(gxr, gyr, offs, n, tid, lane, wave, sH, sRed);
-        hessian_rows<M, P, WPS, 3, 6>(gxr, gyr, offs, n, tid, lane, wave, sH, sRed);
-        hessian_rows<M, P, WPS, 6, 12>(gxr, gyr, offs, n, tid, lane, wave, sH, sRed);
-    } else if constexpr (M == 6) {
-        hessian_rows<M, P, WPS, 0, 2>(gxr, gyr, offs, n, tid, lane, wave, sH, sRed);
-        hessian_rows<M, P, WPS, 2, 6>(gxr, gyr, offs, n, tid, lane, wave, sH, sRed);
+        hessian_rows<M, P, WPS, 0, 2>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+        hessian_rows<M, P, WPS, 2, 5>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+        hessian_rows<M, P, WPS, 5, 12>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
     } else {
-        hessian_rows<M, P, WPS, 0, M>(gxr, gyr, offs, n, tid, lane, wave, sH, sRed);
+        hessian_rows<M, P, WPS, 0, M>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
     }
     __syncthreads();
-    if (tid == 0) sCtl[0] = cholesky<M>(sH, sL) ? 0 : DIC_ERR_SINGULAR_HESSIAN;
+    if (tid == 0) {
+        const bool ok = cholesky<M>(sH, sL);
+        sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
+        if (ok)
+            for (int i = 0; i < M; ++i) sRd[i] = 1.0 / sL[i * (i + 1) / 2 + i];
+    }
     __syncthreads();
     if (sCtl[0]) {
         if (tid == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
         return;
     }
+
+    // ---- diagnostic statistics of the reference (subset.rs:698-727, 786-818):
+    // gradient standardisation, then mean and spread of each projected mode ----
+    {
+        const float mx = (float)(sAS[M] / dn), my = (float)(sAS[M + 1] / dn);
+        float rs[2 * K];
+        float d1[2 + 2 * K];
+#pragma unroll
+        for (int j = 0; j < 2 + 2 * K; ++j) d1[j] = 0.f;
+#pragma unroll 1
+        for (int k = 0; k < P; ++k) {
+            const int i = tid + NT * k;
+            if (i < n) {
+                const float2 d = sXY[i];
+                const float ax = gxr[k] - mx, ay = gyr[k] - my;
+                float mo[4];
+                modes32<M>(d.x, d.y, mo);
+                d1[0] = fmaf(ax, ax, d1[0]);
+                d1[1] = fmaf(ay, ay, d1[1]);
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    d1[2 + q] = fmaf(ax, mo[q], d1[2 + q]);
+                    d1[2 + K + q] = fmaf(ay, mo[q], d1[2 + K + q]);
+                }
+            }
+        }
+        double e1[2 + 2 * K];
+        block_sum32<WPS, 2 + 2 * K>(d1, e1, sRed, lane, wave);
+        const double sdx = sqrt(fmax(e1[0] / dn, 1e-18)), sdy = sqrt(fmax(e1[1] / dn, 1e-18));
+        const float rsdx = (float)(1.0 / sdx), rsdy = (float)(1.0 / sdy);
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            rs[q] = uni((float)(e1[2 + q] / (sdx * dn)));
+            rs[K + q] = uni((float)(e1[2 + K + q] / (sdy * dn)));
+        }
+        float d2[4 * K];
+#pragma unroll
+        for (int j = 0; j < 4 * K; ++j) d2[j] = 0.f;
+#pragma unroll 1
+        for (int k = 0; k < P; ++k) {
+            const int i = tid + NT * k;
+            if (i < n) {
+                const float2 d = sXY[i];
+                const float zx = (gxr[k] - mx) * rsdx, zy = (gyr[k] - my) * rsdy;
+                float mo[4];
+                modes32<M>(d.x, d.y, mo);
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const float ra = fmaf(zx, mo[q], -rs[q]), rb = fmaf(zy, mo[q], -rs[K + q]);
+                    d2[q] = fmaf(ra, ra, d2[q]);
+                    d2[K + q] = fmaf(rb, rb, d2[K + q]);
+                    d2[2 * K + q] += ra;
+                    d2[3 * K + q] += rb;
+                }
+            }
+        }
+        double e2[4 * K];
+        block_sum32<WPS, 4 * K>(d2, e2, sRed, lane, wave);
+        if (tid == 0) {
+            sDg[0] = mx;
+            sDg[1] = my;
+            sDg[2] = rsdx;
+            sDg[3] = rsdy;
+#pragma unroll
+            for (int q = 0; q < 2 * K; ++q) {
+                const double rv = sqrt(fmax(e2[q] / dn, 1e-18));
+                sDg[4 + q] = rs[q];
+                sDg[4 + 2 * K + q] = (float)(1.0 / rv);
+                sDg[4 + 4 * K + q] = (float)(e2[2 * K + q] / rv);  // Σ sc_q
+            }
+        }
+        __syncthreads();
+    }
+
     const double size = sqrt(dn);
-    const float f_m32 = (float)f_m;
+    const float f_m32 = uni((float)f_m);
     double p[M], p_last[M];
 #pragma unroll
     for (int k = 0; k < M; ++k) p_last[k] = p[k] = uni(A.p0[12 * s + k]);
@@ -510,79 +836,125 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : 3)) icgn_batch_kernel(Bat
     bool converged = false;
     int status = 0;
     double g_m = 0.0, delta_g = 0.0, last_norm = 0.0;
+    Window W = no_window();
     for (int it = 1; it <= A.max_it; ++it) {
         double bx0, bx1, by0, by1;
         warp_box<M>(p, ext, bx0, bx1, by0, by1);
-        const Window W = make_window(xc + bx0, xc + bx1, yc + by0, yc + by1, A);
-        if (W.lds) stage_window<NT>(win, A.gC, A.wp, W, tid);
-        __syncthreads();
+        const double ax0 = xc + bx0 - margin, ax1 = xc + bx1 + margin;
+        const double ay0 = yc + by0 - margin, ay1 = yc + by1 + margin;
+        int c[4];
+        const bool box_ok = box_cells(ax0, ax1, ay0, ay1, A, c);
+        if (!box_ok || !covers(W, c)) {
+            // restage: box plus slack, else the bare box, else no window
+            Window Wn = no_window();
+            int cs[4];
+            const double sl = (double)A.win_slack;
+            if (box_ok && box_cells(ax0 - sl, ax1 + sl, ay0 - sl, ay1 + sl, A, cs)) {
+                Wn = window_for(cs, A);
+                if (!Wn.lds) Wn = window_for(c, A);
+            }
+            W = Wn;
+            __syncthreads();  // everyone is done with the previous window
+            if (W.lds) stage_window<NT>(win, A.gC, A.pitch, W, tid);
+            __syncthreads();
+        }
+        const bool fast = box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
         float q[12];
 #pragma unroll
-        for (int k = 0; k < M; ++k) q[k] = (float)p[k];
-        double acc[3 + M];
+        for (int k = 0; k < M; ++k) q[k] = uni((float)p[k]);
+        if (M >= 6) {
+            q[2] = uni((float)(1.0 + p[2]));
+            q[5] = uni((float)(1.0 + p[5]));
+        }
+        if (M == 12) {
 #pragma unroll
-        for (int j = 0; j < 3 + M; ++j) acc[j] = 0.0;
-#pragma unroll 1
-        for (int k = 0; k < P; ++k) {
-            const int i = tid + NT * k;
-            if (i < n) {
-                const int32_t o = offs[i];
-                const float dxf = (float)unpack_dx(o), dyf = (float)unpack_dy(o);
-                float ox, oy;
-                warp_local32<M>(q, fxc, fyc, dxf, dyf, ox, oy);
-                int cx, cy;
-                float tx, ty;
-                cell_frac(bx, ox, A.w, cx, tx);
-                cell_frac(by, oy, A.h, cy, ty);
-                float d0, d1;
-                const float g = eval32<false>(win, A.gC, A.wp, W, A.border, cx, cy, tx, ty, d0, d1);
-                const double gp = (double)(g - f_m32);
-                const double a = (double)am[k];
-                acc[0] += gp;
-                acc[1] = fma(gp, gp, acc[1]);
-                acc[2] = fma(a, gp, acc[2]);
-                double sd[M];
-                sdi64<M>((double)gxr[k], (double)gyr[k], (double)dxf, (double)dyf, sd);
-#pragma unroll
-                for (int j = 0; j < M; ++j) acc[3 + j] = fma(sd[j], gp, acc[3 + j]);
+            for (int k = 6; k < 12; k += 4) {
+                q[k] = uni(0.5f * q[k]);
+                q[k + 1] = uni(0.5f * q[k + 1]);
             }
         }
-        block_sum<WPS, 3 + M>(acc, sRed, lane, wave);
+        const float c0x = uni(fxc + (float)p[0]), c0y = uni(fyc + (float)p[1]);
+        float acc[NA];
+#pragma unroll
+        for (int j = 0; j < NA; ++j) acc[j] = 0.f;
+        if (fast) {
+            const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
+            const float wwf = (float)W.ww;
+#pragma unroll 1
+            for (int k = 0; k < P; ++k) {
+                const int i = tid + NT * k;
+                if (i < n) {
+                    const float2 d = sXY[i];
+                    float ox, oy;
+                    warp_pos<M>(q, c0x, c0y, d.x, d.y, ox, oy);
+                    const float flx = floorf(ox), fly = floorf(oy);
+                    // offsets inside the window are small integers: exact in FP32
+                    float cc[36];
+                    gather6(win + cb + (int)fmaf(fly, wwf, flx), W.ww, cc);
+                    float a[6], b[6];
+                    basis_f32(oy - fly, a);
+                    basis_f32(ox - flx, b);
+                    const float g = contract6r(cc, a, b);
+                    accumulate<M>(acc, g - f_m32, am[k], gxr[k], gyr[k], d.x, d.y);
+                }
+            }
+        } else {
+            // q[2], q[5] hold 1 + u_x, 1 + v_y: the general path uses the
+            // same positions as the fast one
+#pragma unroll 1
+            for (int k = 0; k < P; ++k) {
+                const int i = tid + NT * k;
+                if (i < n) {
+                    const float2 d = sXY[i];
+                    float ox, oy;
+                    warp_pos<M>(q, c0x, c0y, d.x, d.y, ox, oy);
+                    int cx, cy;
+                    float tx, ty;
+                    cell_frac(bx, ox, A.w, cx, tx);
+                    cell_frac(by, oy, A.h, cy, ty);
+                    float d0, d1;
+                    const float g = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, d0, d1);
+                    accumulate<M>(acc, g - f_m32, am[k], gxr[k], gyr[k], d.x, d.y);
+                }
+            }
+        }
+        double sum[NA];
+        block_sum32<WPS, NA>(acc, sum, sRed, lane, wave);
         // g' above is g - (float)f_m; fold the rounding of the shift back in
         const double shift = (double)f_m32 - f_m;
-        const double s1 = acc[0] + dn * shift;
-        const double s2 = acc[1] + shift * (2.0 * acc[0] + dn * shift);
-        const double s3 = acc[2] + shift * sa;
+        const double s1 = sum[0] + dn * shift;
+        const double s2 = sum[1] + shift * (2.0 * sum[0] + dn * shift);
+        const double s3 = sum[2] + shift * sa;
         double Bj[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) Bj[j] = acc[3 + j] + shift * Sj[j];
+        for (int j = 0; j < M; ++j) Bj[j] = sum[3 + j] + shift * sAS[M + j];
         const double mg = s1 / dn;
         g_m = uni(f_m + mg);
         delta_g = uni(sqrt(fmax(s2 - s1 * mg, 0.0)));
         const double ratio = delta_f / delta_g;
         double gz[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) gz[j] = Aj[j] - ratio * (Bj[j] - mg * Sj[j]);
+        for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
         double dp[M];
-        cholesky_solve_neg<M>(sL, gz, dp);
+        chol_solve_neg_rd<M>(sL, sRd, gz, dp);
         double pn[M];
         bool ok;
         if constexpr (M == 12) {
             __syncthreads();
             if (tid == 0) {
-                const bool c = compose_icgn_dev<M>(p, dp, pn, sScr);
-                sCtl[0] = c ? 0 : DIC_ERR_SINGULAR_WARP;
+                const bool cok = compose_icgn_dev<M>(p, dp, pn, sScr);
+                sCtl[0] = cok ? 0 : DIC_ERR_SINGULAR_WARP;
                 for (int k = 0; k < M; ++k) sP[k] = pn[k];
             }
             __syncthreads();
             ok = sCtl[0] == 0;
 #pragma unroll
             for (int k = 0; k < M; ++k) pn[k] = sP[k];
+        } else if constexpr (M == 6) {
+            ok = compose6_fast(p, dp, pn);
         } else {
             ok = compose_icgn_dev<M>(p, dp, pn, sScr);
         }
-        // every thread reads the window before it is restaged
-        __syncthreads();
         if (!ok) {
             status = DIC_ERR_SINGULAR_WARP;
             break;
@@ -615,129 +987,104 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : 3)) icgn_batch_kernel(Bat
         return;
     }
 
-    // ---- final ZNSSD (subset.rs:1190-1193) and warp-adequacy diagnostic
-    // (subset.rs:748-820) on g at the last iteration's warp; with no
-    // iteration g is all zero and the reference's NaN propagates alike ----
-    // e_i = a_i/Δf - (g_i - g_m)/Δg replaces a_i in place: nothing after
-    // this point needs a_i itself, and no extra per-pixel registers are held
-    const double rf = 1.0 / delta_f;
-    const double rg = it_done ? 1.0 / delta_g : qnan;
-    const double gmf = it_done ? g_m : 0.0;
-    double r1[4] = {0.0, 0.0, 0.0, 0.0};  // Σz², Σe, Σgx, Σgy
+    // ---- final ZNSSD (subset.rs:1190-1193) and the diagnostic's projections
+    // on g at the last iteration's warp; with no iteration g is all zero and
+    // the reference's NaN propagates alike ----
+    const float rf = uni((float)(1.0 / delta_f));
+    const float rg = it_done ? uni((float)(1.0 / delta_g)) : __int_as_float(0x7fc00000);
+    const float gm32 = it_done ? uni((float)g_m) : 0.f;
+    float fin[2 + 2 * K];  // Σe², Σe, Σ sc_q·e
+#pragma unroll
+    for (int j = 0; j < 2 + 2 * K; ++j) fin[j] = 0.f;
     {
-        Window W;
-        W.lds = false;
-        W.x0 = W.y0 = W.ww = W.wh = 0;
+        const float mx = sDg[0], my = sDg[1], rsdx = sDg[2], rsdy = sDg[3];
+        float rs[2 * K];
+#pragma unroll
+        for (int q = 0; q < 2 * K; ++q) rs[q] = sDg[4 + q];
+        bool fast = false;
         if (it_done) {
             double bx0, bx1, by0, by1;
             warp_box<M>(p_last, ext, bx0, bx1, by0, by1);
-            W = make_window(xc + bx0, xc + bx1, yc + by0, yc + by1, A);
-            if (W.lds) stage_window<NT>(win, A.gC, A.wp, W, tid);
+            const double ax0 = xc + bx0 - margin, ax1 = xc + bx1 + margin;
+            const double ay0 = yc + by0 - margin, ay1 = yc + by1 + margin;
+            int c[4];
+            const bool box_ok = box_cells(ax0, ax1, ay0, ay1, A, c);
+            if (!box_ok || !covers(W, c)) {
+                W = box_ok ? window_for(c, A) : no_window();
+                __syncthreads();
+                if (W.lds) stage_window<NT>(win, A.gC, A.pitch, W, tid);
+                __syncthreads();
+            }
+            fast = box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
         }
-        __syncthreads();
         float q[12];
 #pragma unroll
-        for (int k = 0; k < M; ++k) q[k] = (float)p_last[k];
+        for (int k = 0; k < M; ++k) q[k] = uni((float)p_last[k]);
+        if (M >= 6) {
+            q[2] = uni((float)(1.0 + p_last[2]));
+            q[5] = uni((float)(1.0 + p_last[5]));
+        }
+        if (M == 12) {
+#pragma unroll
+            for (int k = 6; k < 12; k += 4) {
+                q[k] = uni(0.5f * q[k]);
+                q[k + 1] = uni(0.5f * q[k + 1]);
+            }
+        }
+        const float c0x = uni(fxc + (float)p_last[0]), c0y = uni(fyc + (float)p_last[1]);
+        const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
 #pragma unroll 1
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
+                const float2 d = sXY[i];
                 float g = 0.f;
                 if (it_done) {
-                    const int32_t o = offs[i];
                     float ox, oy;
-                    warp_local32<M>(q, fxc, fyc, (float)unpack_dx(o), (float)unpack_dy(o), ox, oy);
-                    int cx, cy;
-                    float tx, ty;
-                    cell_frac(bx, ox, A.w, cx, tx);
-                    cell_frac(by, oy, A.h, cy, ty);
-                    float d0, d1;
-                    g = eval32<false>(win, A.gC, A.wp, W, A.border, cx, cy, tx, ty, d0, d1);
-                }
-                const double z = (double)am[k] * rf - ((double)g - gmf) * rg;
-                r1[0] = fma(z, z, r1[0]);
-                r1[1] += z;
-                r1[2] += (double)gxr[k];
-                r1[3] += (double)gyr[k];
-                am[k] = (float)z;
-            }
-        }
-    }
-    block_sum<WPS, 4>(r1, sRed, lane, wave);
-    const double znssd = it_done ? r1[0] : 4.0;
-    const double zncc = it_done ? 1.0 - znssd / 2.0 : 0.0;
-    const double e_bar = r1[1] / dn;
-    const double mx = r1[2] / dn, my = r1[3] / dn;
-    double r2[2] = {0.0, 0.0};
-#pragma unroll 1
-    for (int k = 0; k < P; ++k) {
-        const int i = tid + NT * k;
-        if (i < n) {
-            const double a = (double)gxr[k] - mx, b = (double)gyr[k] - my;
-            r2[0] = fma(a, a, r2[0]);
-            r2[1] = fma(b, b, r2[1]);
-        }
-    }
-    block_sum<WPS, 2>(r2, sRed, lane, wave);
-    const double sdx = sqrt(fmax(r2[0] / dn, 1e-18)), sdy = sqrt(fmax(r2[1] / dn, 1e-18));
-    double rs[2 * K], rv[2 * K], rp[2 * K];
-#pragma unroll
-    for (int q = 0; q < 2 * K; ++q) rs[q] = rv[q] = rp[q] = 0.0;
-    for (int pass = 0; pass < 3; ++pass) {
-#pragma unroll 1
-        for (int k = 0; k < P; ++k) {
-            const int i = tid + NT * k;
-            if (i < n) {
-                const int32_t o = offs[i];
-                const double xi = unpack_dx(o), ze = unpack_dy(o);
-                double mo[4];
-                if (M == 2) {
-                    mo[0] = xi;
-                    mo[1] = ze;
-                } else if (M == 6) {
-                    mo[0] = xi * xi;
-                    mo[1] = xi * ze;
-                    mo[2] = ze * ze;
-                } else {
-                    mo[0] = xi * xi * xi;
-                    mo[1] = xi * xi * ze;
-                    mo[2] = xi * ze * ze;
-                    mo[3] = ze * ze * ze;
-                }
-                const double zx = ((double)gxr[k] - mx) / sdx, zy = ((double)gyr[k] - my) / sdy;
-                const double ec = (double)am[k] - e_bar;
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    const double ra = zx * mo[q], rb = zy * mo[q];
-                    if (pass == 0) {
-                        rs[q] += ra;
-                        rs[K + q] += rb;
-                    } else if (pass == 1) {
-                        const double a = ra - rs[q], b = rb - rs[K + q];
-                        rv[q] = fma(a, a, rv[q]);
-                        rv[K + q] = fma(b, b, rv[K + q]);
+                    warp_pos<M>(q, c0x, c0y, d.x, d.y, ox, oy);
+                    if (fast) {
+                        const float flx = floorf(ox), fly = floorf(oy);
+                        float cc[36];
+                        gather6(win + cb + (int)fmaf(fly, (float)W.ww, flx), W.ww, cc);
+                        float a[6], b[6];
+                        basis_f32(oy - fly, a);
+                        basis_f32(ox - flx, b);
+                        g = contract6r(cc, a, b);
                     } else {
-                        rp[q] = fma((ra - rs[q]) / rv[q], ec, rp[q]);
-                        rp[K + q] = fma((rb - rs[K + q]) / rv[K + q], ec, rp[K + q]);
+                        int cx, cy;
+                        float tx, ty;
+                        cell_frac(bx, ox, A.w, cx, tx);
+                        cell_frac(by, oy, A.h, cy, ty);
+                        float d0, d1;
+                        g = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, d0, d1);
                     }
                 }
+                const float e = fmaf(am[k], rf, -(g - gm32) * rg);
+                fin[0] = fmaf(e, e, fin[0]);
+                fin[1] += e;
+                const float zx = (gxr[k] - mx) * rsdx, zy = (gyr[k] - my) * rsdy;
+                float mo[4];
+                modes32<M>(d.x, d.y, mo);
+#pragma unroll
+                for (int qq = 0; qq < K; ++qq) {
+                    fin[2 + qq] = fmaf(fmaf(zx, mo[qq], -rs[qq]), e, fin[2 + qq]);
+                    fin[2 + K + qq] = fmaf(fmaf(zy, mo[qq], -rs[K + qq]), e, fin[2 + K + qq]);
+                }
             }
         }
-        if (pass == 0) {
-            block_sum<WPS, 2 * K>(rs, sRed, lane, wave);
-#pragma unroll
-            for (int q = 0; q < 2 * K; ++q) rs[q] /= dn;
-        } else if (pass == 1) {
-            block_sum<WPS, 2 * K>(rv, sRed, lane, wave);
-#pragma unroll
-            for (int q = 0; q < 2 * K; ++q) rv[q] = sqrt(fmax(rv[q] / dn, 1e-18));
-        }
     }
-    block_sum<WPS, 2 * K>(rp, sRed, lane, wave);
+    double fs[2 + 2 * K];
+    block_sum32<WPS, 2 + 2 * K>(fin, fs, sRed, lane, wave);
+    const double znssd = it_done ? fs[0] : 4.0;
+    const double zncc = it_done ? 1.0 - znssd / 2.0 : 0.0;
+    const double e_bar = fs[1] / dn;
     double eu = 0.0, ev = 0.0;
 #pragma unroll
-    for (int q = 0; q < K; ++q) {
-        const double a = rp[q] / dn, b = rp[K + q] / dn;
+    for (int qq = 0; qq < K; ++qq) {
+        // Σ sc·(e - ē) / n with sc = (z·mode - mean) / spread
+        const double a = ((double)sDg[4 + 2 * K + qq] * fs[2 + qq] - e_bar * (double)sDg[4 + 4 * K + qq]) / dn;
+        const double b =
+            ((double)sDg[4 + 3 * K + qq] * fs[2 + K + qq] - e_bar * (double)sDg[4 + 5 * K + qq]) / dn;
         eu += a * a;
         ev += b * b;
     }

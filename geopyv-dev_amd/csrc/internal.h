@@ -48,6 +48,7 @@ struct Image {
     int64_t h = 0, w = 0;
     int border = 0;
     int64_t hp = 0, wp = 0;  // padded shape
+    int64_t wp32 = 0;        // row pitch of the FP32 copy: wp rounded up to 4 floats (16-byte rows)
     double* coef64 = nullptr;
     float* coef32 = nullptr;
 };

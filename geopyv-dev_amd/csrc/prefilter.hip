@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(1024) prefilter_rows_kernel(const PixT* __rest
 __global__ void __launch_bounds__(1024) prefilter_cols_kernel(const double* __restrict__ tmp, int64_t hp, int64_t wp,
                                                               int npairs, int per_xcd, FftStages st,
                                                               const double2* __restrict__ tw,
-                                                              double* __restrict__ c64, float* __restrict__ c32) {
+                                                              double* __restrict__ c64, float* __restrict__ c32,
+                                                              int64_t wp32) {
     extern __shared__ double2 lds[];
     // XCD-aware: dispatch puts workgroup b on XCD b % 8; give each XCD a
     // contiguous band of column pairs.
@@ -173,11 +174,11 @@ __global__ void __launch_bounds__(1024) prefilter_cols_kernel(const double* __re
         const double2 v = lds[i];
         const double a = v.x * scale;
         c64[i * wp + ca] = a;
-        c32[i * wp + ca] = (float)a;
+        c32[i * wp32 + ca] = (float)a;
         if (hasb) {
             const double bb = v.y * scale;
             c64[i * wp + cb] = bb;
-            c32[i * wp + cb] = (float)bb;
+            c32[i * wp32 + cb] = (float)bb;
         }
     }
 }
@@ -269,7 +270,7 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem)
     DIC_HIP_TRY(hipFuncSetAttribute((const void*)prefilter_cols_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lc));
     prefilter_cols_kernel<<<per_xcd * 8, tc, lc, e->stream>>>(e->pre_tmp, hp, wp, npairs, per_xcd, sc, tw_c,
-                                                              img->coef64, img->coef32);
+                                                              img->coef64, img->coef32, img->wp32);
     DIC_HIP_TRY(hipGetLastError());
     DIC_HIP_TRY(hipEventRecord(e->ev_pre1, e->stream));
     e->pre_recorded = true;

@@ -12,7 +12,8 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libgeodic.so")
+# GEODIC_LIB selects another build of the same library (same C ABI)
+LIB_PATH = os.environ.get("GEODIC_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libgeodic.so")
 
 # status codes (include/geodic.h)
 OK = 0

@@ -67,7 +67,14 @@ int main(int argc, char** argv) {
     std::vector<double> Cf((size_t)HP * WP), Cg((size_t)HP * WP);
     orc_get_c(f.data(), H, W, B, Cf.data(), 1);
     orc_get_c(g.data(), H, W, B, Cg.data(), 1);
-    std::vector<float> Cf32(Cf.begin(), Cf.end()), Cg32(Cg.begin(), Cg.end());
+    // FP32 copies with rows pitched to 4 floats, as the engine stores them
+    const int P32 = (WP + 3) & ~3;
+    std::vector<float> Cf32((size_t)HP * P32, 0.f), Cg32((size_t)HP * P32, 0.f);
+    for (int i = 0; i < HP; ++i)
+        for (int j = 0; j < WP; ++j) {
+            Cf32[(size_t)i * P32 + j] = (float)Cf[(size_t)i * WP + j];
+            Cg32[(size_t)i * P32 + j] = (float)Cg[(size_t)i * WP + j];
+        }
 
     // template and subsets
     const int side = 2 * radius + 1;
@@ -105,7 +112,7 @@ int main(int argc, char** argv) {
     geodic::batch::BatchArgs A{};
     A.fC = Cf32.data();
     A.gC = Cg32.data();
-    A.wp = WP;
+    A.pitch = P32;
     A.border = B;
     A.w = W;
     A.h = H;
@@ -126,9 +133,10 @@ int main(int argc, char** argv) {
     A.max_it = max_it;
     A.tol = 0.75;
     A.max_norm = 1e-5;
-    const int wside = (ext[1] - ext[0] + 1) + 6 + 2 * slack;
-    A.win_cap = wside * wside;
-    const size_t lds = sizeof(float) * A.win_cap;
+    const int wside = (ext[1] - ext[0] + 1) + 2 * slack + 12;
+    A.win_cap = ((wside + 3) & ~3) * wside;
+    A.win_slack = slack;
+    const size_t lds = sizeof(float) * A.win_cap + 2 * sizeof(float) * npx;
     // P = 12 pixels per thread: one wavefront holds 768 pixels
     auto run = [&](auto mtag) {
         constexpr int MM = decltype(mtag)::value;
@@ -136,8 +144,10 @@ int main(int argc, char** argv) {
             simt::launch(n, 64, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 1>(A); });
         else if (wps == 2)
             simt::launch(n, 128, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 2>(A); });
-        else
+        else if (wps == 4)
             simt::launch(n, 256, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 4>(A); });
+        else
+            std::abort();
     };
     if (M == 2)
         run(std::integral_constant<int, 2>{});

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -25,14 +26,29 @@
 #define __launch_bounds__(...)
 #define GEODIC_BATCH_ATTRS(W, OCC)
 #define GEODIC_PLAIN_ATTRS
+#define GEODIC_SCHED_BARRIER()
 #define GEODIC_DYN_LDS(T, name) T* name = (T*)::simt::g_dyn_lds
 
 using std::max;
 using std::min;
 
-struct float4 {
+struct alignas(16) float4 {
     float x, y, z, w;
 };
+struct alignas(8) float2 {
+    float x, y;
+};
+inline float2 make_float2(float x, float y) { return {x, y}; }
+inline float __int_as_float(int v) {
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+inline int __float_as_int(float f) {
+    int v;
+    std::memcpy(&v, &f, 4);
+    return v;
+}
 struct double2 {
     double x, y;
 };
@@ -71,8 +87,10 @@ T xchg(T v, int src) {
 
 // Runs `body` for blocks 0..grid-1, one block (`threads` lanes) at a time.
 inline void launch(int grid, int threads, size_t dyn_lds_bytes, const std::function<void()>& body) {
-    std::vector<unsigned char> lds(dyn_lds_bytes + 64, 0xCD);
-    g_dyn_lds = lds.data();
+    // 16-byte aligned, exactly the requested size (ASan sees overruns)
+    unsigned char* lds = static_cast<unsigned char*>(::operator new(dyn_lds_bytes + 16, std::align_val_t(16)));
+    std::memset(lds, 0xCD, dyn_lds_bytes);
+    g_dyn_lds = lds;
     g_griddim.x = grid;
     g_blockdim.x = threads;
     for (int b = 0; b < grid; ++b) {
@@ -87,6 +105,7 @@ inline void launch(int grid, int threads, size_t dyn_lds_bytes, const std::funct
             });
         for (auto& t : th) t.join();
     }
+    ::operator delete(lds, std::align_val_t(16));
 }
 }  // namespace simt
 
