@@ -215,6 +215,76 @@ __device__ __forceinline__ double wave_sum_u(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// Several wavefront sums at once (transposed tree).  Groups of eight
+// values go through: a permlane32 swap pairing two values (lanes 0-31 then
+// carry the first, 32-63 the second), a permlane16 swap pairing two such
+// registers (one value per 16-lane row), a bank-masked row rotation by 8
+// pairing two of those (one value per 8-lane half-row), and three DPP steps
+// inside each half-row.  Eight sums cost 35 VALU operations instead of 8 x 18
+// for separate trees; leftover values use the single-value tree.  Value
+// 8u + 4h + o of a group ends in lane 16·row(o) + 8h + 4, row = (0, 2, 1, 3)[o].
+// The addition tree is fixed.  Needs every lane active.
+template <int CTRL, int BANK_MASK>
+__device__ __forceinline__ double dpp_merge(double old, double src) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(src), CTRL, 0xf, BANK_MASK, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(src), CTRL, 0xf, BANK_MASK, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double swap32_sum(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                     false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                     false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+__device__ __forceinline__ double swap16_sum(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                     false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                     false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+__device__ __forceinline__ void wave_sums8(const double* x, double* out) {
+    const double r0 = swap32_sum(x[0], x[1]), r1 = swap32_sum(x[2], x[3]);
+    const double r2 = swap32_sum(x[4], x[5]), r3 = swap32_sum(x[6], x[7]);
+    const double t0 = swap16_sum(r0, r1), t1 = swap16_sum(r2, r3);
+    // lanes 0-7 of each row: t0's value; lanes 8-15: t1's
+    double w = dpp_merge<0x128, 0xc>(t0, t1) + dpp_merge<0x128, 0x3>(t1, t0);  // row_ror:8
+    w += dpp_move<0x114, 0xf, true>(w);  // row_shr:4 (lanes 4-7, 12-15 pair correctly)
+    w += dpp_move<0x4e, 0xf, true>(w);   // quad_perm [2,3,0,1]
+    w += dpp_move<0xb1, 0xf, true>(w);   // quad_perm [1,0,3,2]
+    constexpr int kRow[4] = {0, 2, 1, 3};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int lane = 16 * kRow[j & 3] + 8 * (j >> 2) + 4;
+        const int lo = __builtin_amdgcn_readlane(__double2loint(w), lane);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(w), lane);
+        out[j] = __hiloint2double(hi, lo);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void wave_sums(const double* in, double* out) {
+    constexpr int G = V / 8, R = V % 8;
+#pragma unroll
+    for (int u = 0; u < G; ++u) wave_sums8(in + 8 * u, out + 8 * u);
+    if constexpr (R >= 3) {
+        // a padded group is cheaper than three or more single trees
+        double x[8], o[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = j < R ? in[8 * G + j] : 0.0;
+        wave_sums8(x, o);
+#pragma unroll
+        for (int j = 0; j < R; ++j) out[8 * G + j] = o[j];
+    } else {
+#pragma unroll
+        for (int j = 8 * G; j < V; ++j) out[j] = wave_sum_u(in[j]);
+    }
+}
+
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));

@@ -351,28 +351,61 @@ __device__ __forceinline__ float eval32(const float* win, const float* __restric
     return contract6(blk, (int)pitch, a, b);
 }
 
-// Sums V FP64 values over the workgroup: DPP tree per wavefront, then the
-// wavefront partials in wave order through LDS.  Result uniform (SGPRs).
+// Sums V FP64 values over the workgroup: transposed DPP/permlane trees per
+// wavefront, then the wavefront partials in wave order through LDS.  Result
+// uniform (SGPRs) in every wave.
 template <int WPS, int V>
 __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int wave) {
     static_assert(V <= kRedSlots, "reduction buffer");
-#pragma unroll
-    for (int j = 0; j < V; ++j) v[j] = wave_sum_u(v[j]);
+    double w[V];
+    wave_sums<V>(v, w);
     if constexpr (WPS > 1) {
         if (lane == 0) {
 #pragma unroll
-            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = v[j];
+            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < V; ++j) {
             double t = sRed[j];
 #pragma unroll
-            for (int w = 1; w < WPS; ++w) t += sRed[w * kRedSlots + j];
+            for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
             v[j] = uni(t);
         }
         __syncthreads();
+    } else {
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = w[j];
     }
+}
+
+// The same sums delivered to wavefront 0 only (the one that then runs the
+// serial update); returns true there.  Same addition order as block_sum.
+template <int WPS, int V>
+__device__ __forceinline__ bool block_sum_lead(double* v, double* sRed, int lane, int wave) {
+    static_assert(V <= kRedSlots, "reduction buffer");
+    double w[V];
+    wave_sums<V>(v, w);
+    if constexpr (WPS > 1) {
+        if (lane == 0 && wave > 0) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                double t = w[j];
+#pragma unroll
+                for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
+                v[j] = uni(t);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = w[j];
+    }
+    return wave == 0;
 }
 
 // FP32 per-thread partials widened to FP64 and summed over the workgroup.
@@ -571,6 +604,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     __shared__ float sDg[4 + 6 * Modes<M>::K];  // mx, my, 1/sdx, 1/sdy, rs, 1/rv, Σsc
     __shared__ double sScr[M == 12 ? 144 : 1];  // order-2 composition workspace
     __shared__ double sP[12];
+    __shared__ double sG[3];  // g_m, Δg, norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[2];
     const int tid = threadIdx.x;
@@ -919,65 +953,91 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             }
         }
         double sum[NA];
-        block_sum32<WPS, NA>(acc, sum, sRed, lane, wave);
-        // g' above is g - (float)f_m; fold the rounding of the shift back in
-        const double shift = (double)f_m32 - f_m;
-        const double s1 = sum[0] + dn * shift;
-        const double s2 = sum[1] + shift * (2.0 * sum[0] + dn * shift);
-        const double s3 = sum[2] + shift * sa;
-        double Bj[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) Bj[j] = sum[3 + j] + shift * sAS[M + j];
-        const double mg = s1 / dn;
-        g_m = uni(f_m + mg);
-        delta_g = uni(sqrt(fmax(s2 - s1 * mg, 0.0)));
-        const double ratio = delta_f / delta_g;
-        double gz[M];
+        for (int j = 0; j < NA; ++j) sum[j] = (double)acc[j];
+        // the lead wavefront alone runs the update; the others wait at the
+        // barrier below, leaving their SIMD slots to other workgroups
+        if (block_sum_lead<WPS, NA>(sum, sRed, lane, wave)) {
+            // g' above is g - (float)f_m; fold the rounding of the shift back in
+            const double shift = (double)f_m32 - f_m;
+            const double s1 = sum[0] + dn * shift;
+            const double s2 = sum[1] + shift * (2.0 * sum[0] + dn * shift);
+            const double s3 = sum[2] + shift * sa;
+            double Bj[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
-        double dp[M];
-        chol_solve_neg_rd<M>(sL, sRd, gz, dp);
-        double pn[M];
-        bool ok;
-        if constexpr (M == 12) {
-            __syncthreads();
-            if (tid == 0) {
-                const bool cok = compose_icgn_dev<M>(p, dp, pn, sScr);
-                sCtl[0] = cok ? 0 : DIC_ERR_SINGULAR_WARP;
-                for (int k = 0; k < M; ++k) sP[k] = pn[k];
+            for (int j = 0; j < M; ++j) Bj[j] = sum[3 + j] + shift * sAS[M + j];
+            const double mg = s1 / dn;
+            const double gm = f_m + mg;
+            const double dg = sqrt(fmax(s2 - s1 * mg, 0.0));
+            const double ratio = delta_f / dg;
+            double gz[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
+            double dp[M];
+            chol_solve_neg_rd<M>(sL, sRd, gz, dp);
+            double pn[M];
+            bool ok;
+            if constexpr (M == 12) {
+                // one lane composes with the LDS workspace; LDS operations
+                // of a wavefront complete in order
+                if (lane == 0) {
+                    const bool cok = compose_icgn_dev<M>(p, dp, pn, sScr);
+                    sCtl[0] = cok ? 0 : DIC_ERR_SINGULAR_WARP;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) sP[k] = pn[k];
+                }
+                __builtin_amdgcn_wave_barrier();
+                ok = sCtl[0] == 0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) pn[k] = sP[k];
+            } else if constexpr (M == 6) {
+                ok = compose6_fast(p, dp, pn);
+            } else {
+                ok = compose_icgn_dev<M>(p, dp, pn, sScr);
             }
-            __syncthreads();
-            ok = sCtl[0] == 0;
+            const double nrm = convergence_norm_dev<M>(dp, size);
+            if (ok && A.hist && it <= A.hist_cap && lane == 0) {
+                const double znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
+                double* hh = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
+                hh[0] = (double)it;
+                hh[1] = nrm;
+                hh[2] = 1.0 - znssd / 2.0;
+                hh[3] = znssd;
+            }
+            if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < M; ++k) pn[k] = sP[k];
-        } else if constexpr (M == 6) {
-            ok = compose6_fast(p, dp, pn);
-        } else {
-            ok = compose_icgn_dev<M>(p, dp, pn, sScr);
+                for (int k = 0; k < M; ++k) sP[k] = pn[k];
+                sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_WARP;
+                sCtl[1] = (ok && nrm < A.max_norm) ? 1 : 0;
+                sG[0] = gm;
+                sG[1] = dg;
+                sG[2] = nrm;
+            }
         }
-        if (!ok) {
-            status = DIC_ERR_SINGULAR_WARP;
+        if constexpr (WPS > 1) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (sCtl[0]) {
+            status = sCtl[0];
             break;
         }
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             p_last[k] = p[k];
-            p[k] = uni(pn[k]);
+            p[k] = uni(sP[k]);
         }
-        last_norm = uni(convergence_norm_dev<M>(dp, size));
         it_done = it;
-        if (A.hist && it <= A.hist_cap && tid == 0) {
-            const double znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * delta_g);
-            double* hh = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
-            hh[0] = (double)it;
-            hh[1] = last_norm;
-            hh[2] = 1.0 - znssd / 2.0;
-            hh[3] = znssd;
-        }
-        if (last_norm < A.max_norm) {
+        if (sCtl[1]) {
             converged = true;
             break;
         }
+    }
+    if (it_done) {
+        g_m = uni(sG[0]);
+        delta_g = uni(sG[1]);
+        last_norm = uni(sG[2]);
     }
     if (status) {
         if (tid == 0) {

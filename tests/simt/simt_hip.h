@@ -1,9 +1,10 @@
 // CPU emulation of the HIP device subset the engine's kernels use, for
 // checking kernel logic on the host (sanitizers, no GPU).  One block runs at
-// a time as 64 std::threads, one per lane; __syncthreads and every
-// cross-lane operation are barriers, so a shuffle reached by only part of a
-// wavefront deadlocks here instead of silently misbehaving.  Static
-// __shared__ arrays become function statics (one block at a time).
+// a time, one std::thread per lane; __syncthreads is a block barrier and
+// every cross-lane operation a barrier of the lane's wavefront, so a shuffle
+// reached by only part of a wavefront deadlocks here instead of silently
+// misbehaving.  Static __shared__ arrays become function statics (one block
+// at a time).
 #pragma once
 
 #include <algorithm>
@@ -13,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <new>
 #include <thread>
 #include <vector>
@@ -59,6 +61,7 @@ namespace simt {
 constexpr int kLanes = 64;
 constexpr int kMaxThreads = 1024;
 inline std::barrier<>* g_bar = nullptr;
+inline std::barrier<>* g_wave_bar[kMaxThreads / kLanes] = {};
 inline void* g_dyn_lds = nullptr;
 inline uint64_t g_xchg[kMaxThreads];
 struct Idx {
@@ -68,18 +71,19 @@ inline thread_local Idx t_thread;
 inline Idx g_block, g_blockdim{kLanes, 1, 1}, g_griddim{1, 1, 1};
 
 inline void sync() { g_bar->arrive_and_wait(); }
+inline void wave_sync() { g_wave_bar[t_thread.x / kLanes]->arrive_and_wait(); }
 
-// Cross-lane read within the calling thread's wavefront (every thread of
-// the block must reach it: it is a block-wide barrier here).
+// Cross-lane read within the calling thread's wavefront (every lane of the
+// wavefront must reach it: it is a wavefront barrier here).
 template <class T>
 T xchg(T v, int src) {
     static_assert(sizeof(T) <= 8, "exchange slot");
     uint64_t bits = 0;
     std::memcpy(&bits, &v, sizeof(T));
     g_xchg[t_thread.x] = bits;
-    sync();
+    wave_sync();
     uint64_t o = g_xchg[(t_thread.x & ~(kLanes - 1)) | (src & (kLanes - 1))];
-    sync();
+    wave_sync();
     T r;
     std::memcpy(&r, &o, sizeof(T));
     return r;
@@ -97,6 +101,11 @@ inline void launch(int grid, int threads, size_t dyn_lds_bytes, const std::funct
         g_block.x = b;
         std::barrier<> bar(threads);
         g_bar = &bar;
+        std::vector<std::unique_ptr<std::barrier<>>> wb;
+        for (int w = 0; w < (threads + kLanes - 1) / kLanes; ++w) {
+            wb.emplace_back(new std::barrier<>(std::min(kLanes, threads - w * kLanes)));
+            g_wave_bar[w] = wb.back().get();
+        }
         std::vector<std::thread> th;
         for (int l = 0; l < threads; ++l)
             th.emplace_back([l, &body] {
@@ -115,6 +124,7 @@ inline void launch(int grid, int threads, size_t dyn_lds_bytes, const std::funct
 #define gridDim (::simt::g_griddim)
 
 inline void __syncthreads() { simt::sync(); }
+inline void __builtin_amdgcn_wave_barrier() { simt::wave_sync(); }
 template <class T>
 T __shfl_xor(T v, int off, int = 64) {
     return simt::xchg(v, simt::t_thread.x ^ off);
@@ -127,11 +137,11 @@ inline int __builtin_amdgcn_readfirstlane(int v) { return simt::xchg(v, 0); }
 inline int __builtin_amdgcn_readlane(int v, int lane) { return simt::xchg(v, lane); }
 
 // DPP move on gfx9 encodings used by the kernels: quad_perm (0x00-0xff),
-// row_shl:n (0x101-0x10f), row_shr:n (0x111-0x11f), row_bcast:15 (0x142),
-// row_bcast:31 (0x143).  Lanes of rows outside row_mask keep `old`; a lane
-// whose source is outside its row reads 0 with bound_ctrl, else keeps old.
+// row_shl:n (0x101-0x10f), row_shr:n (0x111-0x11f), row_ror:n (0x121-0x12f),
+// row_bcast:15 (0x142), row_bcast:31 (0x143).  Lanes of rows outside
+// row_mask, or of 4-lane banks outside bank_mask, keep `old`; a lane whose
+// source is outside its row reads 0 with bound_ctrl, else keeps old.
 inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
-    (void)bank_mask;
     const int lane = simt::t_thread.x & 63;
     const int row = lane >> 4, rl = lane & 15;
     int from = -1;
@@ -143,6 +153,9 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
     } else if (ctrl >= 0x111 && ctrl <= 0x11f) {
         const int n = ctrl & 15;
         from = (rl >= n) ? lane - n : -1;
+    } else if (ctrl >= 0x121 && ctrl <= 0x12f) {
+        const int n = ctrl & 15;
+        from = 16 * row + ((rl - n + 16) & 15);
     } else if (ctrl == 0x142) {
         from = row >= 1 ? 16 * row - 1 : -1;
     } else if (ctrl == 0x143) {
@@ -152,8 +165,36 @@ inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask,
     }
     const int v = simt::xchg(src, from < 0 ? lane : from);  // xchg stays within the wavefront
     if (!((row_mask >> row) & 1)) return old;
+    if (!((bank_mask >> (rl >> 2)) & 1)) return old;
     if (from < 0) return bound_ctrl ? 0 : old;
     return v;
+}
+
+// v_permlane32_swap / v_permlane16_swap: element 0 is the new VDST, element
+// 1 the new VSRC0.  32: VDST lanes 32-63 <-> VSRC0 lanes 0-31.  16: VDST's
+// odd 16-lane rows <-> VSRC0's even rows.
+struct simt_u32x2 {
+    unsigned v[2];
+    unsigned operator[](int i) const { return v[i]; }
+};
+inline simt_u32x2 __builtin_amdgcn_permlane32_swap(unsigned dst, unsigned src, bool, bool) {
+    const int lane = simt::t_thread.x & 63;
+    const unsigned d_peer = simt::xchg(dst, lane ^ 32);
+    const unsigned s_peer = simt::xchg(src, lane ^ 32);
+    simt_u32x2 r;
+    r.v[0] = lane < 32 ? dst : s_peer;  // upper VDST lanes take VSRC0's lower lanes
+    r.v[1] = lane < 32 ? d_peer : src;  // lower VSRC0 lanes take VDST's upper lanes
+    return r;
+}
+inline simt_u32x2 __builtin_amdgcn_permlane16_swap(unsigned dst, unsigned src, bool, bool) {
+    const int lane = simt::t_thread.x & 63;
+    const unsigned d_peer = simt::xchg(dst, lane ^ 16);
+    const unsigned s_peer = simt::xchg(src, lane ^ 16);
+    const bool odd = (lane >> 4) & 1;
+    simt_u32x2 r;
+    r.v[0] = odd ? s_peer : dst;   // odd VDST rows take VSRC0's even rows
+    r.v[1] = odd ? src : d_peer;   // even VSRC0 rows take VDST's odd rows
+    return r;
 }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline double __longlong_as_double(long long v) {
