@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--waves-per-subset", type=int, default=0, help="batched-engine schedule (0 = automatic)")
     ap.add_argument("--win-slack", type=int, default=3, help="batched-engine LDS window slack in pixels")
+    ap.add_argument("--max-iterations", type=int, default=50,
+                    help="iteration cap (the workload's is 50; smaller values split the frame time by phase)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (known HBM bytes) for counter scaling")
     ap.add_argument("--engine", choices=["both", "f32"], default="both",
@@ -108,8 +110,8 @@ def main():
 
     def frame(precision):
         g_img.update(device_ptr=dev_tar.data_ptr(), pix_type=_native.PIX_U8)
-        batch.solve(f_img, g_img, method="icgn", precision=precision, max_iterations=50, tolerance=0.75,
-                    max_norm=1e-5)
+        batch.solve(f_img, g_img, method="icgn", precision=precision, max_iterations=args.max_iterations,
+                    tolerance=0.75, max_norm=1e-5)
         eng.sync()
 
     # ---- FP32-interpolation batched engine (the measured value) ----
@@ -132,7 +134,8 @@ def main():
 
     # ---- plain FP64 drop-in on the same frames ----
     plain_lat, plain_sol = [], []
-    if args.engine == "f32":
+    if args.engine == "f32" or args.max_iterations != 50:
+        # profiler and phase-timing runs: no headline line
         args.plain_steps = 0
         args.no_cpu = True
     else:

@@ -144,8 +144,8 @@ __device__ __forceinline__ void eval_grad_f64(const double* __restrict__ C, int6
 // (b_{5-k}(t) = b_k(1-t)) splits into an even part E and an odd part O,
 // b_k = E + O, b_{5-k} = E - O, each a quadratic in z: 23 operations for the
 // six weights instead of 28 in power form.
-__device__ __forceinline__ void basis_f32(float t, float b[6]) {
-    const float s = t - 0.5f, z = s * s;
+__device__ __forceinline__ void basis_f32_s(float s, float b[6]) {
+    const float z = s * s;
     const float e0 = fmaf(fmaf(z, 1.f / 48.f, 1.f / 96.f), z, 1.f / 3840.f);
     const float o0 = s * fmaf(fmaf(z, -1.f / 120.f, -1.f / 48.f), z, -1.f / 384.f);
     const float e1 = fmaf(fmaf(z, -1.f / 16.f, 7.f / 32.f), z, 79.f / 1280.f);
@@ -159,6 +159,8 @@ __device__ __forceinline__ void basis_f32(float t, float b[6]) {
     b[2] = e2 + o2;
     b[3] = e2 - o2;
 }
+
+__device__ __forceinline__ void basis_f32(float t, float b[6]) { basis_f32_s(t - 0.5f, b); }
 
 __device__ __forceinline__ void dbasis_f32(float t, float d[6]) {
     d[0] = (float)GQ10 + t * (2.f * (float)GQ20 + t * (3.f * (float)GQ30 + t * (4.f * (float)GQ40 + t * 5.f * (float)GQ50)));

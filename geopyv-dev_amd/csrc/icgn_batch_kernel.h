@@ -186,11 +186,11 @@ __device__ __forceinline__ bool covers(const Window& W, const int c[4]) {
     return W.lds && c[0] >= W.cx_lo && c[1] <= W.cx_hi && c[2] >= W.cy_lo && c[3] <= W.cy_hi;
 }
 
-// Stage the window into LDS: thread t copies 16-byte chunks t, t + NT, ...
-// of the row-major chunk grid, four loads in flight.
+// Stage the window into LDS, less `shift`: thread t copies 16-byte chunks
+// t, t + NT, ... of the row-major chunk grid, four loads in flight.
 template <int NT>
 __device__ __forceinline__ void stage_window(float* win, const float* __restrict__ C, int64_t pitch, const Window& W,
-                                             int tid) {
+                                             int tid, float shift) {
     const int cpr = W.ww >> 2;  // chunks per row
     const int total = cpr * W.wh;
     const float* __restrict__ src0 = C + (int64_t)W.y0 * pitch + W.x0;
@@ -216,10 +216,13 @@ __device__ __forceinline__ void stage_window(float* win, const float* __restrict
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[j] = *(const float4*)(src0 + rr[j] * pi + 4 * cc[j]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) *(float4*)(win + rr[j] * W.ww + 4 * cc[j]) = v[j];
+        for (int j = 0; j < 4; ++j)
+            *(float4*)(win + rr[j] * W.ww + 4 * cc[j]) =
+                make_float4(v[j].x - shift, v[j].y - shift, v[j].z - shift, v[j].w - shift);
     }
     for (; k < total; k += NT) {
-        *(float4*)(win + r * W.ww + 4 * c) = *(const float4*)(src0 + r * pi + 4 * c);
+        const float4 v = *(const float4*)(src0 + r * pi + 4 * c);
+        *(float4*)(win + r * W.ww + 4 * c) = make_float4(v.x - shift, v.y - shift, v.z - shift, v.w - shift);
         r += dr;
         c += dc;
         if (c >= cpr) {
@@ -327,12 +330,15 @@ __device__ __forceinline__ float contract6_grad(const float* blk, int stride, co
     return val;
 }
 
-// General gather at cell (cx, cy) + (tx, ty): LDS when the block is inside
-// the window, HBM otherwise (two explicit paths so each compiles to its own
-// address space).
+// General gather at cell (cx, cy) + (tx, ty), less `shift`: LDS when the
+// block is inside the window (staged already less `shift`), HBM otherwise
+// (two explicit paths so each compiles to its own address space).  The
+// basis sums to one, so subtracting the shift from the value equals
+// subtracting it from every coefficient.
 template <bool GRAD>
 __device__ __forceinline__ float eval32(const float* win, const float* __restrict__ C, int64_t pitch, const Window& W,
-                                        int border, int cellx, int celly, float tx, float ty, float& gx, float& gy) {
+                                        int border, int cellx, int celly, float tx, float ty, float shift, float& gx,
+                                        float& gy) {
     float a[6], b[6], da[6], db[6];
     basis_f32(ty, a);
     basis_f32(tx, b);
@@ -347,8 +353,8 @@ __device__ __forceinline__ float eval32(const float* win, const float* __restric
         return contract6(blk, W.ww, a, b);
     }
     const float* __restrict__ blk = C + (int64_t)(celly + border - 2) * pitch + (cellx + border - 2);
-    if (GRAD) return contract6_grad(blk, (int)pitch, a, b, da, db, gx, gy);
-    return contract6(blk, (int)pitch, a, b);
+    if (GRAD) return contract6_grad(blk, (int)pitch, a, b, da, db, gx, gy) - shift;
+    return contract6(blk, (int)pitch, a, b) - shift;
 }
 
 // Sums V FP64 values over the workgroup: transposed DPP/permlane trees per
@@ -588,25 +594,116 @@ __device__ __forceinline__ bool compose6_fast(const double* p, const double* dp,
     return true;
 }
 
+// What one iteration needs, decided by the lead wavefront from the FP64
+// warp and published through LDS: the window to use (and whether to stage
+// it first), whether the fast path applies, and the FP32 warp.
+struct Plan {
+    int x0, y0, ww, wh, cx_lo, cx_hi, cy_lo, cy_hi;
+    int lds, restage, fast, pad;
+    float q[12];  // q[2] = 1 + u_x, q[5] = 1 + v_y, second-order terms pre-halved
+    float c0x, c0y;  // centre fraction + (u, v): the warped centre relative to the base
+};
+
+template <int M>
+__device__ __forceinline__ Plan make_plan(const double* p, const int32_t ext[4], double xc, double yc, float fxc,
+                                          float fyc, const Window& Wcur, const BatchArgs& A) {
+    const double margin = 1e-3;  // covers the FP32 rounding of per-pixel positions
+    double bx0, bx1, by0, by1;
+    warp_box<M>(p, ext, bx0, bx1, by0, by1);
+    const double ax0 = xc + bx0 - margin, ax1 = xc + bx1 + margin;
+    const double ay0 = yc + by0 - margin, ay1 = yc + by1 + margin;
+    int c[4];
+    const bool box_ok = box_cells(ax0, ax1, ay0, ay1, A, c);
+    Window W = Wcur;
+    bool restage = false;
+    if (!box_ok) {
+        W = no_window();
+    } else if (!covers(Wcur, c)) {
+        // box plus slack, else the bare box, else no window
+        int cs[4];
+        const double sl = (double)A.win_slack;
+        W = no_window();
+        if (box_cells(ax0 - sl, ax1 + sl, ay0 - sl, ay1 + sl, A, cs)) W = window_for(cs, A);
+        if (!W.lds) W = window_for(c, A);
+        restage = W.lds;
+    }
+    Plan P;
+    P.x0 = W.x0;
+    P.y0 = W.y0;
+    P.ww = W.ww;
+    P.wh = W.wh;
+    P.cx_lo = W.cx_lo;
+    P.cx_hi = W.cx_hi;
+    P.cy_lo = W.cy_lo;
+    P.cy_hi = W.cy_hi;
+    P.lds = W.lds ? 1 : 0;
+    P.restage = restage ? 1 : 0;
+    P.fast = (box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax) ? 1 : 0;
+    P.pad = 0;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) P.q[k] = (k < M) ? (float)p[k] : 0.f;
+    if (M >= 6) {
+        P.q[2] = (float)(1.0 + p[2]);
+        P.q[5] = (float)(1.0 + p[5]);
+    }
+    if (M == 12) {
+        P.q[6] = (float)(0.5 * p[6]);
+        P.q[7] = (float)(0.5 * p[7]);
+        P.q[10] = (float)(0.5 * p[10]);
+        P.q[11] = (float)(0.5 * p[11]);
+    }
+    P.c0x = fxc + (float)p[0];
+    P.c0y = fyc + (float)p[1];
+    return P;
+}
+
+__device__ __forceinline__ Window plan_window(const Plan& P) {
+    Window W;
+    W.x0 = P.x0;
+    W.y0 = P.y0;
+    W.ww = P.ww;
+    W.wh = P.wh;
+    W.cx_lo = P.cx_lo;
+    W.cx_hi = P.cx_hi;
+    W.cy_lo = P.cy_lo;
+    W.cy_hi = P.cy_hi;
+    W.lds = P.lds != 0;
+    return W;
+}
+
+// Plan read back from LDS into scalar registers.
+__device__ __forceinline__ Plan load_plan(const Plan* sp) {
+    Plan P;
+    const int* si = (const int*)sp;
+    int* di = (int*)&P;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(Plan) / 4); ++k) di[k] = uni(si[k]);
+    return P;
+}
+
 template <int M, int P, int WPS>
 __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_batch_kernel(BatchArgs A) {
     constexpr int NT = 64 * WPS;
     constexpr int NL = M * (M + 1) / 2;
     constexpr int K = Modes<M>::K;
-    constexpr int NA = 3 + M;  // per-iteration sums
+    constexpr int NA = 3 + M;                          // per-iteration sums
+    constexpr int NH1 = (M <= 6) ? NL : 0;             // Hessian entries summed in the first setup pass
+    constexpr int V1 = 2 + 2 * M + NH1;                // first setup pass
+    constexpr int V2 = 4 + 4 * K;                      // second setup pass
     GEODIC_DYN_LDS(float, lds);
     float* win = lds;
     float2* sXY = (float2*)(lds + A.win_cap);
     __shared__ double sH[M * M];
     __shared__ double sL[NL];
     __shared__ double sRd[M];
-    __shared__ double sAS[2 * M];           // A_j, S_j
-    __shared__ float sDg[4 + 6 * Modes<M>::K];  // mx, my, 1/sdx, 1/sdy, rs, 1/rv, Σsc
+    __shared__ double sAS[2 * M];               // A_j, S_j
+    __shared__ float sDg[4 + 4 * Modes<M>::K];  // mx, my, 1/sdx, 1/sdy, rs, 1/rv
     __shared__ double sScr[M == 12 ? 144 : 1];  // order-2 composition workspace
     __shared__ double sP[12];
     __shared__ double sG[3];  // g_m, Δg, norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[2];
+    __shared__ Plan sPlan;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const double qnan = __longlong_as_double(0x7ff8000000000000LL);
@@ -659,7 +756,10 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
 
     float am[P], gxr[P], gyr[P];
 
-    // ---- Subset::new on the staged reference window ----
+    // ---- Subset::new on the staged reference window: one pass evaluates f
+    // and ∇f and sums everything that does not need f_m, using the
+    // provisional shift c: Σ sdi_j (f - f_m) = Σ sdi_j (f - c) - (f_m - c) S_j ----
+    const float cshift = 127.5f;
     double f_m, delta_f, sa;
     {
         const double ax0 = xc + ext[0] - margin, ax1 = xc + ext[1] + margin;
@@ -667,197 +767,181 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         int c[4];
         Window W = no_window();
         if (box_cells(ax0, ax1, ay0, ay1, A, c)) W = window_for(c, A);
-        if (W.lds) stage_window<NT>(win, A.fC, A.pitch, W, tid);
+        if (W.lds) stage_window<NT>(win, A.fC, A.pitch, W, tid, 0.f);
         __syncthreads();
         const bool fast = W.lds && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
-        float red[2] = {0.f, 0.f};  // Σf, SSSIG
-        if (fast) {
-            // every pixel sits at an integer offset from the centre: one
-            // basis for the whole subset
-            float ba[6], bb[6], bda[6], bdb[6];
-            basis_f32(fyc, ba);
-            basis_f32(fxc, bb);
-            dbasis_f32(fyc, bda);
-            dbasis_f32(fxc, bdb);
+        float u[V1];
 #pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                ba[q] = uni(ba[q]);
-                bb[q] = uni(bb[q]);
-                bda[q] = uni(bda[q]);
-                bdb[q] = uni(bdb[q]);
-            }
-            const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
-            const float wwf = (float)W.ww;
+        for (int j = 0; j < V1; ++j) u[j] = 0.f;
+        // every pixel sits at an integer offset from the centre: in the fast
+        // path one basis serves the whole subset
+        float ba[6], bb[6], bda[6], bdb[6];
+        basis_f32(fyc, ba);
+        basis_f32(fxc, bb);
+        dbasis_f32(fyc, bda);
+        dbasis_f32(fxc, bdb);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            ba[q] = uni(ba[q]);
+            bb[q] = uni(bb[q]);
+            bda[q] = uni(bda[q]);
+            bdb[q] = uni(bdb[q]);
+        }
+        const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
+        const float wwf = (float)W.ww;
 #pragma unroll 1
-            for (int k = 0; k < P; ++k) {
-                const int i = tid + NT * k;
-                float v = 0.f, gx = 0.f, gy = 0.f;
-                if (i < n) {
-                    const float2 d = sXY[i];
+        for (int k = 0; k < P; ++k) {
+            const int i = tid + NT * k;
+            float v = 0.f, gx = 0.f, gy = 0.f;
+            if (i < n) {
+                const float2 d = sXY[i];
+                if (fast) {
                     float cc[36];
                     gather6(win + cb + (int)fmaf(d.y, wwf, d.x), W.ww, cc);
                     v = contract6r_grad(cc, ba, bb, bda, bdb, gx, gy);
-                    red[0] += v;
-                    red[1] = fmaf(0.5f, fmaf(gx, gx, gy * gy), red[1]);
-                }
-                am[k] = v;
-                gxr[k] = gx;
-                gyr[k] = gy;
-            }
-        } else {
-#pragma unroll 1
-            for (int k = 0; k < P; ++k) {
-                const int i = tid + NT * k;
-                float v = 0.f, gx = 0.f, gy = 0.f;
-                if (i < n) {
-                    const float2 d = sXY[i];
+                } else {
                     int cx, cy;
                     float tx, ty;
                     cell_frac(bx, fxc + d.x, A.w, cx, tx);
                     cell_frac(by, fyc + d.y, A.h, cy, ty);
-                    v = eval32<true>(win, A.fC, A.pitch, W, A.border, cx, cy, tx, ty, gx, gy);
-                    red[0] += v;
-                    red[1] = fmaf(0.5f, fmaf(gx, gx, gy * gy), red[1]);
+                    v = eval32<true>(win, A.fC, A.pitch, W, A.border, cx, cy, tx, ty, 0.f, gx, gy);
                 }
-                am[k] = v;
-                gxr[k] = gx;
-                gyr[k] = gy;
+                const float a = v - cshift;
+                u[0] += v;
+                u[1] = fmaf(0.5f, fmaf(gx, gx, gy * gy), u[1]);
+                float sd[M];
+                sdi32<M>(gx, gy, d.x, d.y, sd);
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    u[2 + j] = fmaf(sd[j], a, u[2 + j]);
+                    u[2 + M + j] += sd[j];
+                }
+                if constexpr (NH1 > 0) {
+                    int q = 2 + 2 * M;
+#pragma unroll
+                    for (int aa = 0; aa < M; ++aa)
+#pragma unroll
+                        for (int b2 = aa; b2 < M; ++b2) {
+                            u[q] = fmaf(sd[aa], sd[b2], u[q]);
+                            ++q;
+                        }
+                }
+                v = a;
+            }
+            am[k] = v;
+            gxr[k] = gx;
+            gyr[k] = gy;
+        }
+        double w1[V1];
+        block_sum32<WPS, V1>(u, w1, sRed, lane, wave);
+        f_m = uni(w1[0] / dn);
+        const double dlt = f_m - (double)cshift;
+        if (tid == 0) {
+            R->f_m = f_m;
+            R->sssig = w1[1];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                sAS[j] = w1[2 + j] - dlt * w1[2 + M + j];
+                sAS[M + j] = w1[2 + M + j];
+            }
+            if constexpr (NH1 > 0) {
+                int q = 2 + 2 * M;
+                for (int aa = 0; aa < M; ++aa)
+                    for (int b2 = aa; b2 < M; ++b2) {
+                        sH[aa * M + b2] = w1[q];
+                        sH[b2 * M + aa] = w1[q];
+                        ++q;
+                    }
             }
         }
-        double rd[2];
-        block_sum32<WPS, 2>(red, rd, sRed, lane, wave);
-        f_m = uni(rd[0] / dn);
-        const double ss = rd[1];
-        float v2[2 + 2 * M];
+        if constexpr (M == 12) {
+            hessian_rows<M, P, WPS, 0, 2>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+            hessian_rows<M, P, WPS, 2, 5>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+            hessian_rows<M, P, WPS, 5, 12>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+        }
+        // the lead lane factorises (its status reaches every wavefront
+        // through the barriers of the next reduction) ...
+        if (tid == 0) {
+            const bool ok = cholesky<M>(sH, sL);
+            sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
+            if (ok)
+                for (int i = 0; i < M; ++i) sRd[i] = 1.0 / sL[i * (i + 1) / 2 + i];
+        }
+        // ... and the lead wavefront plans the first iteration
+        if (wave == 0) {
+            double p0[M];
 #pragma unroll
-        for (int j = 0; j < 2 + 2 * M; ++j) v2[j] = 0.f;
+            for (int k = 0; k < M; ++k) p0[k] = uni(A.p0[12 * s + k]);
+            const Plan pl = make_plan<M>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
+            if (lane == 0) sPlan = pl;
+        }
+
+        // ---- second pass: a = f - f_m in place, and the diagnostic's
+        // reference statistics (subset.rs:698-727, 786-818): gradient
+        // spreads, mode means and mode spreads ----
+        const float d32 = (float)dlt;
+        const float mx = (float)(w1[2 + M] / dn), my = (float)(w1[3 + M] / dn);
+        float v2[V2];
+#pragma unroll
+        for (int j = 0; j < V2; ++j) v2[j] = 0.f;
 #pragma unroll 1
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
                 const float2 d = sXY[i];
-                const float a = (float)((double)am[k] - f_m);
-                v2[0] = fmaf(a, a, v2[0]);
-                v2[1] += a;
-                float sd[M];
-                sdi32<M>(gxr[k], gyr[k], d.x, d.y, sd);
-#pragma unroll
-                for (int j = 0; j < M; ++j) {
-                    v2[2 + j] = fmaf(sd[j], a, v2[2 + j]);
-                    v2[2 + M + j] += sd[j];
-                }
+                const float a = am[k] - d32;
                 am[k] = a;
+                v2[0] += a;
+                v2[1] = fmaf(a, a, v2[1]);
+                const float ax = gxr[k] - mx, ay = gyr[k] - my;
+                const float ax2 = ax * ax, ay2 = ay * ay;
+                v2[2] += ax2;
+                v2[3] += ay2;
+                float mo[4];
+                modes32<M>(d.x, d.y, mo);
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const float m2 = mo[q] * mo[q];
+                    v2[4 + q] = fmaf(ax, mo[q], v2[4 + q]);
+                    v2[4 + K + q] = fmaf(ay, mo[q], v2[4 + K + q]);
+                    v2[4 + 2 * K + q] = fmaf(ax2, m2, v2[4 + 2 * K + q]);
+                    v2[4 + 3 * K + q] = fmaf(ay2, m2, v2[4 + 3 * K + q]);
+                }
             }
         }
-        double w2[2 + 2 * M];
-        block_sum32<WPS, 2 + 2 * M>(v2, w2, sRed, lane, wave);
-        const double sq = w2[0];
-        sa = w2[1];
+        double w2[V2];
+        block_sum32<WPS, V2>(v2, w2, sRed, lane, wave);
+        sa = w2[0];
+        const double sq = w2[1];
         delta_f = uni(sqrt(sq));
         if (tid == 0) {
-#pragma unroll
-            for (int j = 0; j < 2 * M; ++j) sAS[j] = w2[2 + j];
-            R->f_m = f_m;
             R->delta_f = delta_f;
-            R->sssig = ss;
             R->sigma = sqrt(sq / dn);
+            const double sdx = sqrt(fmax(w2[2] / dn, 1e-18)), sdy = sqrt(fmax(w2[3] / dn, 1e-18));
+            sDg[0] = mx;
+            sDg[1] = my;
+            sDg[2] = (float)(1.0 / sdx);
+            sDg[3] = (float)(1.0 / sdy);
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                // mean of z·mode, and its spread from Σ(z·mode)² - n·mean²
+                const double rsx = w2[4 + q] / (sdx * dn), rsy = w2[4 + K + q] / (sdy * dn);
+                const double vx = w2[4 + 2 * K + q] / (sdx * sdx * dn) - rsx * rsx;
+                const double vy = w2[4 + 3 * K + q] / (sdy * sdy * dn) - rsy * rsy;
+                sDg[4 + q] = (float)rsx;
+                sDg[4 + K + q] = (float)rsy;
+                sDg[4 + 2 * K + q] = (float)(1.0 / sqrt(fmax(vx, 1e-18)));
+                sDg[4 + 3 * K + q] = (float)(1.0 / sqrt(fmax(vy, 1e-18)));
+            }
         }
     }
     if (delta_f == 0.0) {
         if (tid == 0) R->status = DIC_ERR_INVALID_INPUT;
         return;
     }
-    if constexpr (M == 12) {
-        hessian_rows<M, P, WPS, 0, 2>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
-        hessian_rows<M, P, WPS, 2, 5>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
-        hessian_rows<M, P, WPS, 5, 12>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
-    } else {
-        hessian_rows<M, P, WPS, 0, M>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const bool ok = cholesky<M>(sH, sL);
-        sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
-        if (ok)
-            for (int i = 0; i < M; ++i) sRd[i] = 1.0 / sL[i * (i + 1) / 2 + i];
-    }
-    __syncthreads();
     if (sCtl[0]) {
         if (tid == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
         return;
-    }
-
-    // ---- diagnostic statistics of the reference (subset.rs:698-727, 786-818):
-    // gradient standardisation, then mean and spread of each projected mode ----
-    {
-        const float mx = (float)(sAS[M] / dn), my = (float)(sAS[M + 1] / dn);
-        float rs[2 * K];
-        float d1[2 + 2 * K];
-#pragma unroll
-        for (int j = 0; j < 2 + 2 * K; ++j) d1[j] = 0.f;
-#pragma unroll 1
-        for (int k = 0; k < P; ++k) {
-            const int i = tid + NT * k;
-            if (i < n) {
-                const float2 d = sXY[i];
-                const float ax = gxr[k] - mx, ay = gyr[k] - my;
-                float mo[4];
-                modes32<M>(d.x, d.y, mo);
-                d1[0] = fmaf(ax, ax, d1[0]);
-                d1[1] = fmaf(ay, ay, d1[1]);
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    d1[2 + q] = fmaf(ax, mo[q], d1[2 + q]);
-                    d1[2 + K + q] = fmaf(ay, mo[q], d1[2 + K + q]);
-                }
-            }
-        }
-        double e1[2 + 2 * K];
-        block_sum32<WPS, 2 + 2 * K>(d1, e1, sRed, lane, wave);
-        const double sdx = sqrt(fmax(e1[0] / dn, 1e-18)), sdy = sqrt(fmax(e1[1] / dn, 1e-18));
-        const float rsdx = (float)(1.0 / sdx), rsdy = (float)(1.0 / sdy);
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            rs[q] = uni((float)(e1[2 + q] / (sdx * dn)));
-            rs[K + q] = uni((float)(e1[2 + K + q] / (sdy * dn)));
-        }
-        float d2[4 * K];
-#pragma unroll
-        for (int j = 0; j < 4 * K; ++j) d2[j] = 0.f;
-#pragma unroll 1
-        for (int k = 0; k < P; ++k) {
-            const int i = tid + NT * k;
-            if (i < n) {
-                const float2 d = sXY[i];
-                const float zx = (gxr[k] - mx) * rsdx, zy = (gyr[k] - my) * rsdy;
-                float mo[4];
-                modes32<M>(d.x, d.y, mo);
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    const float ra = fmaf(zx, mo[q], -rs[q]), rb = fmaf(zy, mo[q], -rs[K + q]);
-                    d2[q] = fmaf(ra, ra, d2[q]);
-                    d2[K + q] = fmaf(rb, rb, d2[K + q]);
-                    d2[2 * K + q] += ra;
-                    d2[3 * K + q] += rb;
-                }
-            }
-        }
-        double e2[4 * K];
-        block_sum32<WPS, 4 * K>(d2, e2, sRed, lane, wave);
-        if (tid == 0) {
-            sDg[0] = mx;
-            sDg[1] = my;
-            sDg[2] = rsdx;
-            sDg[3] = rsdy;
-#pragma unroll
-            for (int q = 0; q < 2 * K; ++q) {
-                const double rv = sqrt(fmax(e2[q] / dn, 1e-18));
-                sDg[4 + q] = rs[q];
-                sDg[4 + 2 * K + q] = (float)(1.0 / rv);
-                sDg[4 + 4 * K + q] = (float)(e2[2 * K + q] / rv);  // Σ sc_q
-            }
-        }
-        __syncthreads();
     }
 
     const double size = sqrt(dn);
@@ -870,48 +954,22 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     bool converged = false;
     int status = 0;
     double g_m = 0.0, delta_g = 0.0, last_norm = 0.0;
-    Window W = no_window();
+    Plan pl = load_plan(&sPlan);
     for (int it = 1; it <= A.max_it; ++it) {
-        double bx0, bx1, by0, by1;
-        warp_box<M>(p, ext, bx0, bx1, by0, by1);
-        const double ax0 = xc + bx0 - margin, ax1 = xc + bx1 + margin;
-        const double ay0 = yc + by0 - margin, ay1 = yc + by1 + margin;
-        int c[4];
-        const bool box_ok = box_cells(ax0, ax1, ay0, ay1, A, c);
-        if (!box_ok || !covers(W, c)) {
-            // restage: box plus slack, else the bare box, else no window
-            Window Wn = no_window();
-            int cs[4];
-            const double sl = (double)A.win_slack;
-            if (box_ok && box_cells(ax0 - sl, ax1 + sl, ay0 - sl, ay1 + sl, A, cs)) {
-                Wn = window_for(cs, A);
-                if (!Wn.lds) Wn = window_for(c, A);
-            }
-            W = Wn;
-            __syncthreads();  // everyone is done with the previous window
-            if (W.lds) stage_window<NT>(win, A.gC, A.pitch, W, tid);
+        if (it > 1) pl = load_plan(&sPlan);
+        const Window W = plan_window(pl);
+        if (pl.restage) {
+            // the target window holds C - f_m, so gathers give g' = g - f_m
+            stage_window<NT>(win, A.gC, A.pitch, W, tid, f_m32);
             __syncthreads();
         }
-        const bool fast = box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
-        float q[12];
-#pragma unroll
-        for (int k = 0; k < M; ++k) q[k] = uni((float)p[k]);
-        if (M >= 6) {
-            q[2] = uni((float)(1.0 + p[2]));
-            q[5] = uni((float)(1.0 + p[5]));
-        }
-        if (M == 12) {
-#pragma unroll
-            for (int k = 6; k < 12; k += 4) {
-                q[k] = uni(0.5f * q[k]);
-                q[k + 1] = uni(0.5f * q[k + 1]);
-            }
-        }
-        const float c0x = uni(fxc + (float)p[0]), c0y = uni(fyc + (float)p[1]);
         float acc[NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) acc[j] = 0.f;
-        if (fast) {
+        if (pl.fast) {
+            // positions less 1/2, rounded to nearest: the cell and the
+            // centred basis coordinate s in [-1/2, 1/2] in two operations
+            const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
             const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
             const float wwf = (float)W.ww;
 #pragma unroll 1
@@ -920,35 +978,32 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 if (i < n) {
                     const float2 d = sXY[i];
                     float ox, oy;
-                    warp_pos<M>(q, c0x, c0y, d.x, d.y, ox, oy);
-                    const float flx = floorf(ox), fly = floorf(oy);
+                    warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
+                    const float rx = rintf(ox), ry = rintf(oy);
                     // offsets inside the window are small integers: exact in FP32
                     float cc[36];
-                    gather6(win + cb + (int)fmaf(fly, wwf, flx), W.ww, cc);
+                    gather6(win + cb + (int)fmaf(ry, wwf, rx), W.ww, cc);
                     float a[6], b[6];
-                    basis_f32(oy - fly, a);
-                    basis_f32(ox - flx, b);
-                    const float g = contract6r(cc, a, b);
-                    accumulate<M>(acc, g - f_m32, am[k], gxr[k], gyr[k], d.x, d.y);
+                    basis_f32_s(oy - ry, a);
+                    basis_f32_s(ox - rx, b);
+                    accumulate<M>(acc, contract6r(cc, a, b), am[k], gxr[k], gyr[k], d.x, d.y);
                 }
             }
         } else {
-            // q[2], q[5] hold 1 + u_x, 1 + v_y: the general path uses the
-            // same positions as the fast one
 #pragma unroll 1
             for (int k = 0; k < P; ++k) {
                 const int i = tid + NT * k;
                 if (i < n) {
                     const float2 d = sXY[i];
                     float ox, oy;
-                    warp_pos<M>(q, c0x, c0y, d.x, d.y, ox, oy);
+                    warp_pos<M>(pl.q, pl.c0x, pl.c0y, d.x, d.y, ox, oy);
                     int cx, cy;
                     float tx, ty;
                     cell_frac(bx, ox, A.w, cx, tx);
                     cell_frac(by, oy, A.h, cy, ty);
                     float d0, d1;
-                    const float g = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, d0, d1);
-                    accumulate<M>(acc, g - f_m32, am[k], gxr[k], gyr[k], d.x, d.y);
+                    const float gp = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, f_m32, d0, d1);
+                    accumulate<M>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
                 }
             }
         }
@@ -989,13 +1044,14 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 __builtin_amdgcn_wave_barrier();
                 ok = sCtl[0] == 0;
 #pragma unroll
-                for (int k = 0; k < M; ++k) pn[k] = sP[k];
+                for (int k = 0; k < M; ++k) pn[k] = uni(sP[k]);
             } else if constexpr (M == 6) {
                 ok = compose6_fast(p, dp, pn);
             } else {
                 ok = compose_icgn_dev<M>(p, dp, pn, sScr);
             }
             const double nrm = convergence_norm_dev<M>(dp, size);
+            const bool conv = ok && nrm < A.max_norm;
             if (ok && A.hist && it <= A.hist_cap && lane == 0) {
                 const double znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
                 double* hh = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
@@ -1004,14 +1060,22 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 hh[2] = 1.0 - znssd / 2.0;
                 hh[3] = znssd;
             }
+            // plan the next iteration while the other wavefronts wait
+            Plan nx = pl;
+            if (ok && !conv && it < A.max_it) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
+                nx = make_plan<M>(pn, ext, xc, yc, fxc, fyc, W, A);
+            }
             if (lane == 0) {
 #pragma unroll
                 for (int k = 0; k < M; ++k) sP[k] = pn[k];
                 sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_WARP;
-                sCtl[1] = (ok && nrm < A.max_norm) ? 1 : 0;
+                sCtl[1] = conv ? 1 : 0;
                 sG[0] = gm;
                 sG[1] = dg;
                 sG[2] = nrm;
+                sPlan = nx;
             }
         }
         if constexpr (WPS > 1) {
@@ -1034,11 +1098,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             break;
         }
     }
-    if (it_done) {
-        g_m = uni(sG[0]);
-        delta_g = uni(sG[1]);
-        last_norm = uni(sG[2]);
-    }
     if (status) {
         if (tid == 0) {
             R->status = status;
@@ -1046,14 +1105,20 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         }
         return;
     }
+    if (it_done) {
+        g_m = uni(sG[0]);
+        delta_g = uni(sG[1]);
+        last_norm = uni(sG[2]);
+    }
 
     // ---- final ZNSSD (subset.rs:1190-1193) and the diagnostic's projections
-    // on g at the last iteration's warp; with no iteration g is all zero and
-    // the reference's NaN propagates alike ----
+    // on g at the last iteration's warp (its plan and staged window are still
+    // current); with no iteration g is all zero and the reference's NaN
+    // propagates alike ----
     const float rf = uni((float)(1.0 / delta_f));
     const float rg = it_done ? uni((float)(1.0 / delta_g)) : __int_as_float(0x7fc00000);
-    const float gm32 = it_done ? uni((float)g_m) : 0.f;
-    float fin[2 + 2 * K];  // Σe², Σe, Σ sc_q·e
+    const float mg32 = it_done ? uni((float)(g_m - f_m)) : -f_m32;
+    float fin[2 + 2 * K];  // Σe², Σe, Σ (z·mode - mean)·e
 #pragma unroll
     for (int j = 0; j < 2 + 2 * K; ++j) fin[j] = 0.f;
     {
@@ -1061,65 +1126,37 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         float rs[2 * K];
 #pragma unroll
         for (int q = 0; q < 2 * K; ++q) rs[q] = sDg[4 + q];
-        bool fast = false;
-        if (it_done) {
-            double bx0, bx1, by0, by1;
-            warp_box<M>(p_last, ext, bx0, bx1, by0, by1);
-            const double ax0 = xc + bx0 - margin, ax1 = xc + bx1 + margin;
-            const double ay0 = yc + by0 - margin, ay1 = yc + by1 + margin;
-            int c[4];
-            const bool box_ok = box_cells(ax0, ax1, ay0, ay1, A, c);
-            if (!box_ok || !covers(W, c)) {
-                W = box_ok ? window_for(c, A) : no_window();
-                __syncthreads();
-                if (W.lds) stage_window<NT>(win, A.gC, A.pitch, W, tid);
-                __syncthreads();
-            }
-            fast = box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
-        }
-        float q[12];
-#pragma unroll
-        for (int k = 0; k < M; ++k) q[k] = uni((float)p_last[k]);
-        if (M >= 6) {
-            q[2] = uni((float)(1.0 + p_last[2]));
-            q[5] = uni((float)(1.0 + p_last[5]));
-        }
-        if (M == 12) {
-#pragma unroll
-            for (int k = 6; k < 12; k += 4) {
-                q[k] = uni(0.5f * q[k]);
-                q[k + 1] = uni(0.5f * q[k + 1]);
-            }
-        }
-        const float c0x = uni(fxc + (float)p_last[0]), c0y = uni(fyc + (float)p_last[1]);
+        const Window W = plan_window(pl);
+        const bool fast = it_done && pl.fast;
+        const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
         const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
 #pragma unroll 1
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
                 const float2 d = sXY[i];
-                float g = 0.f;
-                if (it_done) {
+                float gp = 0.f;  // g - f_m
+                if (fast) {
                     float ox, oy;
-                    warp_pos<M>(q, c0x, c0y, d.x, d.y, ox, oy);
-                    if (fast) {
-                        const float flx = floorf(ox), fly = floorf(oy);
-                        float cc[36];
-                        gather6(win + cb + (int)fmaf(fly, (float)W.ww, flx), W.ww, cc);
-                        float a[6], b[6];
-                        basis_f32(oy - fly, a);
-                        basis_f32(ox - flx, b);
-                        g = contract6r(cc, a, b);
-                    } else {
-                        int cx, cy;
-                        float tx, ty;
-                        cell_frac(bx, ox, A.w, cx, tx);
-                        cell_frac(by, oy, A.h, cy, ty);
-                        float d0, d1;
-                        g = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, d0, d1);
-                    }
+                    warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
+                    const float rx = rintf(ox), ry = rintf(oy);
+                    float cc[36];
+                    gather6(win + cb + (int)fmaf(ry, (float)W.ww, rx), W.ww, cc);
+                    float a[6], b[6];
+                    basis_f32_s(oy - ry, a);
+                    basis_f32_s(ox - rx, b);
+                    gp = contract6r(cc, a, b);
+                } else if (it_done) {
+                    float ox, oy;
+                    warp_pos<M>(pl.q, pl.c0x, pl.c0y, d.x, d.y, ox, oy);
+                    int cx, cy;
+                    float tx, ty;
+                    cell_frac(bx, ox, A.w, cx, tx);
+                    cell_frac(by, oy, A.h, cy, ty);
+                    float d0, d1;
+                    gp = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, f_m32, d0, d1);
                 }
-                const float e = fmaf(am[k], rf, -(g - gm32) * rg);
+                const float e = fmaf(am[k], rf, -(gp - mg32) * rg);
                 fin[0] = fmaf(e, e, fin[0]);
                 fin[1] += e;
                 const float zx = (gxr[k] - mx) * rsdx, zy = (gyr[k] - my) * rsdy;
@@ -1135,20 +1172,18 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     }
     double fs[2 + 2 * K];
     block_sum32<WPS, 2 + 2 * K>(fin, fs, sRed, lane, wave);
-    const double znssd = it_done ? fs[0] : 4.0;
-    const double zncc = it_done ? 1.0 - znssd / 2.0 : 0.0;
-    const double e_bar = fs[1] / dn;
-    double eu = 0.0, ev = 0.0;
-#pragma unroll
-    for (int qq = 0; qq < K; ++qq) {
-        // Σ sc·(e - ē) / n with sc = (z·mode - mean) / spread
-        const double a = ((double)sDg[4 + 2 * K + qq] * fs[2 + qq] - e_bar * (double)sDg[4 + 4 * K + qq]) / dn;
-        const double b =
-            ((double)sDg[4 + 3 * K + qq] * fs[2 + K + qq] - e_bar * (double)sDg[4 + 5 * K + qq]) / dn;
-        eu += a * a;
-        ev += b * b;
-    }
     if (tid == 0) {
+        const double znssd = it_done ? fs[0] : 4.0;
+        const double zncc = it_done ? 1.0 - znssd / 2.0 : 0.0;
+        // Σ sc·(e - ē) with sc = (z·mode - mean)/spread; Σ sc = 0, so the
+        // centring of e drops out
+        double eu = 0.0, ev = 0.0;
+        for (int qq = 0; qq < K; ++qq) {
+            const double a = (double)sDg[4 + 2 * K + qq] * fs[2 + qq] / dn;
+            const double b = (double)sDg[4 + 3 * K + qq] * fs[2 + K + qq] / dn;
+            eu += a * a;
+            ev += b * b;
+        }
         for (int k = 0; k < 12; ++k) {
             R->p[k] = (k < M) ? p[k] : 0.0;
             A.p_out[12 * s + k] = (k < M) ? p[k] : 0.0;
@@ -1159,8 +1194,9 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         R->iterations = it_done;
         R->converged = converged ? 1 : 0;
         R->quality_ok = (converged && zncc >= A.tol) ? 1 : 0;
-        R->eta_u = sqrt(eu);
-        R->eta_v = sqrt(ev);
+        // with no iteration e is NaN (g_m, Δg undefined), as in the reference
+        R->eta_u = it_done ? sqrt(eu) : qnan;
+        R->eta_v = it_done ? sqrt(ev) : qnan;
     }
 }
 

@@ -3,7 +3,8 @@
 // FP64 oracle.  Built with AddressSanitizer/UBSan by tests/simt/Makefile.
 //
 //   simt_driver ORDER RADIUS NSIDE MAXIT [WIN_SLACK] [WPS]
-// exit 0 when every subset matches (|du|,|dv| < 1e-4 px), 1 otherwise.
+// exit 0 when every subset matches (status, convergence, |du|,|dv| < 1e-4 px,
+// ZNCC within 1e-6, η within 1 % + 1e-6), 1 otherwise.
 #include "simt_hip.h"
 
 #include <cstdio>
@@ -169,7 +170,10 @@ int main(int argc, char** argv) {
         for (int k = 2; k < M; ++k) ds = std::max(ds, std::fabs(res[s].p[k] - orc[s].p[k]));
         dumax = std::max(dumax, du);
         dsmax = std::max(dsmax, ds);
-        const bool ok = res[s].status == orc[s].status && du < 1e-4 && res[s].converged == orc[s].converged;
+        const double deta = std::max(std::fabs(res[s].eta_u - orc[s].eta_u), std::fabs(res[s].eta_v - orc[s].eta_v));
+        const double eta_tol = 1e-6 + 1e-2 * std::max(std::fabs(orc[s].eta_u), std::fabs(orc[s].eta_v));
+        const bool ok = res[s].status == orc[s].status && du < 1e-4 && res[s].converged == orc[s].converged &&
+                        std::fabs(res[s].c_zncc - orc[s].c_zncc) < 1e-6 && deta < eta_tol;
         if (!ok) ++bad;
         printf("s=%d st=%d/%d it=%d/%d conv=%d/%d u=%.8f/%.8f v=%.8f/%.8f zncc=%.9f/%.9f eta=%.3e/%.3e\n", s,
                res[s].status, orc[s].status, res[s].iterations, orc[s].iterations, res[s].converged, orc[s].converged,
