@@ -10,20 +10,27 @@ using batch::BatchArgs;
 using batch::icgn_batch_kernel;
 using batch::kRedSlots;
 
-template <int M, int P, int WPS>
+template <int M, int P, int WPS, int S>
 int launch_mpw(Batch* b, BatchArgs& A, int blocks, size_t lds) {
     Engine* e = b->eng;
-    DIC_HIP_TRY(hipFuncSetAttribute((const void*)icgn_batch_kernel<M, P, WPS>,
+    DIC_HIP_TRY(hipFuncSetAttribute((const void*)icgn_batch_kernel<M, P, WPS, S>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    icgn_batch_kernel<M, P, WPS><<<blocks, 64 * WPS, lds, e->stream>>>(A);
+    icgn_batch_kernel<M, P, WPS, S><<<blocks, 64 * WPS, lds, e->stream>>>(A);
     DIC_HIP_TRY(hipGetLastError());
     return DIC_OK;
+}
+
+// LDS row stride: 64 floats (rows addressed by immediate offsets) when every
+// window of the batch fits it, else the window's own width.
+template <int M, int WPS>
+int launch_ms(Batch* b, BatchArgs& A, int blocks, size_t lds, bool s64) {
+    return s64 ? launch_mpw<M, 12, WPS, 64>(b, A, blocks, lds) : launch_mpw<M, 12, WPS, 0>(b, A, blocks, lds);
 }
 
 // Enough wavefronts per subset to hold the largest subset at 12 pixels per
 // thread (the footprint that stays in registers without spilling).
 template <int M>
-int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds) {
+int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds, bool s64) {
     const int n = b->max_n;
     int wps = b->waves_per_subset;
     if (wps <= 0) wps = n <= 64 * 12 ? 1 : (n <= 128 * 12 ? 2 : (n <= 256 * 12 ? 4 : 8));
@@ -32,10 +39,10 @@ int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds) {
         return DIC_ERR_INVALID_INPUT;
     }
     switch (wps) {
-        case 1: return launch_mpw<M, 12, 1>(b, A, blocks, lds);
-        case 2: return launch_mpw<M, 12, 2>(b, A, blocks, lds);
-        case 4: return launch_mpw<M, 12, 4>(b, A, blocks, lds);
-        case 8: return launch_mpw<M, 12, 8>(b, A, blocks, lds);
+        case 1: return launch_ms<M, 1>(b, A, blocks, lds, s64);
+        case 2: return launch_ms<M, 2>(b, A, blocks, lds, s64);
+        case 4: return launch_ms<M, 4>(b, A, blocks, lds, s64);
+        case 8: return launch_ms<M, 8>(b, A, blocks, lds, s64);
         default: break;
     }
     set_error("invalid input: waves per subset must be 1, 2, 4 or 8");
@@ -73,14 +80,16 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     // rounding and the restaging slack, with room for moderate strain)
     // followed by the template offsets as FP32 pairs
     const int side = b->max_side + 2 * b->win_slack + 12;
-    A.win_cap = ((side + 3) & ~3) * side;
+    const int row = (side + 3) & ~3;
+    const bool s64 = row <= 64;
+    A.win_cap = (s64 ? 64 : row) * side;
     A.win_slack = b->win_slack;
     const size_t lds = sizeof(float) * (size_t)A.win_cap + 2 * sizeof(float) * (size_t)b->max_n;
     const int blocks = (int)b->n;
     const int order = b->max_order;
-    if (order == 0) return launch_m<2>(b, A, blocks, lds);
-    if (order == 1) return launch_m<6>(b, A, blocks, lds);
-    return launch_m<12>(b, A, blocks, lds);
+    if (order == 0) return launch_m<2>(b, A, blocks, lds, s64);
+    if (order == 1) return launch_m<6>(b, A, blocks, lds, s64);
+    return launch_m<12>(b, A, blocks, lds, s64);
 }
 
 }  // namespace geodic

@@ -42,6 +42,8 @@
 //    projections Σ sc_q·e together (Σ sc_q·(e - ē) = Σ sc_q·e - ē Σ sc_q).
 #pragma once
 
+#include <cstddef>
+
 #include "bspline.h"
 #include "linalg.h"
 #include "../../include/geodic.h"
@@ -130,18 +132,18 @@ __device__ __forceinline__ void warp_box(const double* p, const int32_t e[4], do
 }
 
 // Staged coefficient window: rows [y0, y0 + wh), columns [x0, x0 + ww) of
-// C, row stride ww floats in LDS (x0 and ww multiples of 4).  Pixels whose
+// C (x0 and ww multiples of 4), row stride ls floats in LDS.  Pixels whose
 // cell lies in [cx_lo, cx_hi] x [cy_lo, cy_hi] find their whole 6x6 block
 // inside.  lds == false: nothing staged, every gather reads HBM.
 struct Window {
-    int x0, y0, ww, wh;
+    int x0, y0, ww, wh, ls;
     int cx_lo, cx_hi, cy_lo, cy_hi;
     bool lds;
 };
 
 __device__ __forceinline__ Window no_window() {
     Window W;
-    W.x0 = W.y0 = W.ww = W.wh = 0;
+    W.x0 = W.y0 = W.ww = W.wh = W.ls = 0;
     W.cx_lo = W.cy_lo = 1;
     W.cx_hi = W.cy_hi = 0;
     W.lds = false;
@@ -152,6 +154,14 @@ __device__ __forceinline__ Window no_window() {
 // (image.rs:130-131); false for a NaN box.
 __device__ __forceinline__ bool box_cells(double ax0, double ax1, double ay0, double ay1, const BatchArgs& A,
                                           int c[4]) {
+    if (ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax && ax0 <= ax1 && ay0 <= ay1) {
+        // inside the image: the clamp is the identity
+        c[0] = uni((int)floor(ax0));
+        c[1] = uni((int)floor(ax1));
+        c[2] = uni((int)floor(ay0));
+        c[3] = uni((int)floor(ay1));
+        return true;
+    }
     ax0 = clamp_ref(ax0, 0.0, A.wmax);
     ax1 = clamp_ref(ax1, 0.0, A.wmax);
     ay0 = clamp_ref(ay0, 0.0, A.hmax);
@@ -166,7 +176,10 @@ __device__ __forceinline__ bool box_cells(double ax0, double ax1, double ay0, do
 
 // Window holding the blocks of cells [c0, c1] x [c2, c3] (all in the image).
 // Columns round out to 16-byte boundaries; the FP32 rows are pitched to
-// 16 bytes, so x0 + ww never passes the pitch.
+// 16 bytes, so x0 + ww never passes the pitch.  S > 0 fixes the LDS row
+// stride at compile time (gathers then address rows by immediate offsets);
+// a wider window does not fit such a layout.
+template <int S>
 __device__ __forceinline__ Window window_for(const int c[4], const BatchArgs& A) {
     Window W;
     W.x0 = (c[0] + A.border - 2) & ~3;
@@ -178,7 +191,8 @@ __device__ __forceinline__ Window window_for(const int c[4], const BatchArgs& A)
     W.cx_hi = W.x0 + W.ww - A.border - 4;
     W.cy_lo = c[2];
     W.cy_hi = c[3];
-    W.lds = W.ww * W.wh <= A.win_cap;
+    W.ls = S > 0 ? S : W.ww;
+    W.lds = W.ww <= W.ls && W.ls * W.wh <= A.win_cap;
     return W;
 }
 
@@ -217,12 +231,12 @@ __device__ __forceinline__ void stage_window(float* win, const float* __restrict
         for (int j = 0; j < 4; ++j) v[j] = *(const float4*)(src0 + rr[j] * pi + 4 * cc[j]);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            *(float4*)(win + rr[j] * W.ww + 4 * cc[j]) =
+            *(float4*)(win + rr[j] * W.ls + 4 * cc[j]) =
                 make_float4(v[j].x - shift, v[j].y - shift, v[j].z - shift, v[j].w - shift);
     }
     for (; k < total; k += NT) {
         const float4 v = *(const float4*)(src0 + r * pi + 4 * c);
-        *(float4*)(win + r * W.ww + 4 * c) = make_float4(v.x - shift, v.y - shift, v.z - shift, v.w - shift);
+        *(float4*)(win + r * W.ls + 4 * c) = make_float4(v.x - shift, v.y - shift, v.z - shift, v.w - shift);
         r += dr;
         c += dc;
         if (c >= cpr) {
@@ -273,6 +287,33 @@ __device__ __forceinline__ void gather6(const float* blk, int stride, float c[36
 #pragma unroll
         for (int k = 0; k < 6; ++k) c[6 * r + k] = blk[r * stride + k];
     GEODIC_SCHED_BARRIER();
+}
+
+// The same with a compile-time row stride S: rows 0-3 are immediate offsets
+// from blk, rows 4-5 from blk4 = blk + 4 S (DS offsets reach 255 dwords).
+// blk4 is passed in, computed from the run-time stride, so the compiler
+// keeps one second base instead of re-deriving an address per read.
+template <int S>
+__device__ __forceinline__ void gather6s(const float* blk, const float* blk4, float c[36]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c[6 * r + k] = blk[r * S + k];
+#pragma unroll
+    for (int r = 4; r < 6; ++r)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c[6 * r + k] = blk4[(r - 4) * S + k];
+    GEODIC_SCHED_BARRIER();
+}
+
+// Gather a block at LDS index idx with row stride S (compile time) or ls.
+template <int S>
+__device__ __forceinline__ void gather_block(const float* win, int idx, int ls, float c[36]) {
+    if constexpr (S > 0) {
+        gather6s<S>(win + idx, win + idx + 4 * ls, c);
+    } else {
+        gather6(win + idx, ls, c);
+    }
 }
 
 __device__ __forceinline__ float contract6r(const float c[36], const float a[6], const float b[6]) {
@@ -348,9 +389,9 @@ __device__ __forceinline__ float eval32(const float* win, const float* __restric
     }
     const bool in = W.lds && cellx >= W.cx_lo && cellx <= W.cx_hi && celly >= W.cy_lo && celly <= W.cy_hi;
     if (in) {
-        const float* blk = win + (celly - W.cy_lo) * W.ww + (cellx - W.cx_lo);
-        if (GRAD) return contract6_grad(blk, W.ww, a, b, da, db, gx, gy);
-        return contract6(blk, W.ww, a, b);
+        const float* blk = win + (celly - W.cy_lo) * W.ls + (cellx - W.cx_lo);
+        if (GRAD) return contract6_grad(blk, W.ls, a, b, da, db, gx, gy);
+        return contract6(blk, W.ls, a, b);
     }
     const float* __restrict__ blk = C + (int64_t)(celly + border - 2) * pitch + (cellx + border - 2);
     if (GRAD) return contract6_grad(blk, (int)pitch, a, b, da, db, gx, gy) - shift;
@@ -599,12 +640,12 @@ __device__ __forceinline__ bool compose6_fast(const double* p, const double* dp,
 // it first), whether the fast path applies, and the FP32 warp.
 struct Plan {
     int x0, y0, ww, wh, cx_lo, cx_hi, cy_lo, cy_hi;
-    int lds, restage, fast, pad;
+    int ls, lds, restage, fast;
     float q[12];  // q[2] = 1 + u_x, q[5] = 1 + v_y, second-order terms pre-halved
     float c0x, c0y;  // centre fraction + (u, v): the warped centre relative to the base
 };
 
-template <int M>
+template <int M, int S>
 __device__ __forceinline__ Plan make_plan(const double* p, const int32_t ext[4], double xc, double yc, float fxc,
                                           float fyc, const Window& Wcur, const BatchArgs& A) {
     const double margin = 1e-3;  // covers the FP32 rounding of per-pixel positions
@@ -623,8 +664,8 @@ __device__ __forceinline__ Plan make_plan(const double* p, const int32_t ext[4],
         int cs[4];
         const double sl = (double)A.win_slack;
         W = no_window();
-        if (box_cells(ax0 - sl, ax1 + sl, ay0 - sl, ay1 + sl, A, cs)) W = window_for(cs, A);
-        if (!W.lds) W = window_for(c, A);
+        if (box_cells(ax0 - sl, ax1 + sl, ay0 - sl, ay1 + sl, A, cs)) W = window_for<S>(cs, A);
+        if (!W.lds) W = window_for<S>(c, A);
         restage = W.lds;
     }
     Plan P;
@@ -632,6 +673,7 @@ __device__ __forceinline__ Plan make_plan(const double* p, const int32_t ext[4],
     P.y0 = W.y0;
     P.ww = W.ww;
     P.wh = W.wh;
+    P.ls = W.ls;
     P.cx_lo = W.cx_lo;
     P.cx_hi = W.cx_hi;
     P.cy_lo = W.cy_lo;
@@ -639,7 +681,6 @@ __device__ __forceinline__ Plan make_plan(const double* p, const int32_t ext[4],
     P.lds = W.lds ? 1 : 0;
     P.restage = restage ? 1 : 0;
     P.fast = (box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax) ? 1 : 0;
-    P.pad = 0;
 #pragma unroll
     for (int k = 0; k < 12; ++k) P.q[k] = (k < M) ? (float)p[k] : 0.f;
     if (M >= 6) {
@@ -663,6 +704,7 @@ __device__ __forceinline__ Window plan_window(const Plan& P) {
     W.y0 = P.y0;
     W.ww = P.ww;
     W.wh = P.wh;
+    W.ls = P.ls;
     W.cx_lo = P.cx_lo;
     W.cx_hi = P.cx_hi;
     W.cy_lo = P.cy_lo;
@@ -671,17 +713,21 @@ __device__ __forceinline__ Window plan_window(const Plan& P) {
     return W;
 }
 
-// Plan read back from LDS into scalar registers.
+// Plan read back from LDS: the integers into scalar registers, the warp as
+// plain (lane-invariant) loads.
 __device__ __forceinline__ Plan load_plan(const Plan* sp) {
     Plan P;
     const int* si = (const int*)sp;
     int* di = (int*)&P;
+    constexpr int NI = (int)(offsetof(Plan, q) / 4);
 #pragma unroll
-    for (int k = 0; k < (int)(sizeof(Plan) / 4); ++k) di[k] = uni(si[k]);
+    for (int k = 0; k < NI; ++k) di[k] = uni(si[k]);
+#pragma unroll
+    for (int k = NI; k < (int)(sizeof(Plan) / 4); ++k) di[k] = si[k];
     return P;
 }
 
-template <int M, int P, int WPS>
+template <int M, int P, int WPS, int S>
 __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_batch_kernel(BatchArgs A) {
     constexpr int NT = 64 * WPS;
     constexpr int NL = M * (M + 1) / 2;
@@ -766,7 +812,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         const double ay0 = yc + ext[2] - margin, ay1 = yc + ext[3] + margin;
         int c[4];
         Window W = no_window();
-        if (box_cells(ax0, ax1, ay0, ay1, A, c)) W = window_for(c, A);
+        if (box_cells(ax0, ax1, ay0, ay1, A, c)) W = window_for<S>(c, A);
         if (W.lds) stage_window<NT>(win, A.fC, A.pitch, W, tid, 0.f);
         __syncthreads();
         const bool fast = W.lds && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
@@ -787,8 +833,9 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             bda[q] = uni(bda[q]);
             bdb[q] = uni(bdb[q]);
         }
-        const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
-        const float wwf = (float)W.ww;
+        const int LS = S > 0 ? S : W.ls;
+        const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
+        const float wwf = (float)LS;
 #pragma unroll 1
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
@@ -797,7 +844,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 const float2 d = sXY[i];
                 if (fast) {
                     float cc[36];
-                    gather6(win + cb + (int)fmaf(d.y, wwf, d.x), W.ww, cc);
+                    gather_block<S>(win, cb + (int)fmaf(d.y, wwf, d.x), W.ls, cc);
                     v = contract6r_grad(cc, ba, bb, bda, bdb, gx, gy);
                 } else {
                     int cx, cy;
@@ -872,7 +919,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             double p0[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) p0[k] = uni(A.p0[12 * s + k]);
-            const Plan pl = make_plan<M>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
+            const Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
             if (lane == 0) sPlan = pl;
         }
 
@@ -945,6 +992,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     }
 
     const double size = sqrt(dn);
+    const double rn = 1.0 / dn;
+    const double max_norm2 = A.max_norm * A.max_norm;
     const float f_m32 = uni((float)f_m);
     double p[M], p_last[M];
 #pragma unroll
@@ -970,8 +1019,9 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             // positions less 1/2, rounded to nearest: the cell and the
             // centred basis coordinate s in [-1/2, 1/2] in two operations
             const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
-            const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
-            const float wwf = (float)W.ww;
+            const int LS = S > 0 ? S : W.ls;
+            const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
+            const float wwf = (float)LS;
 #pragma unroll 1
             for (int k = 0; k < P; ++k) {
                 const int i = tid + NT * k;
@@ -982,7 +1032,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                     const float rx = rintf(ox), ry = rintf(oy);
                     // offsets inside the window are small integers: exact in FP32
                     float cc[36];
-                    gather6(win + cb + (int)fmaf(ry, wwf, rx), W.ww, cc);
+                    gather_block<S>(win, cb + (int)fmaf(ry, wwf, rx), W.ls, cc);
                     float a[6], b[6];
                     basis_f32_s(oy - ry, a);
                     basis_f32_s(ox - rx, b);
@@ -1021,7 +1071,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             double Bj[M];
 #pragma unroll
             for (int j = 0; j < M; ++j) Bj[j] = sum[3 + j] + shift * sAS[M + j];
-            const double mg = s1 / dn;
+            const double mg = s1 * rn;
             const double gm = f_m + mg;
             const double dg = sqrt(fmax(s2 - s1 * mg, 0.0));
             const double ratio = delta_f / dg;
@@ -1050,9 +1100,11 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             } else {
                 ok = compose_icgn_dev<M>(p, dp, pn, sScr);
             }
-            const double nrm = convergence_norm_dev<M>(dp, size);
-            const bool conv = ok && nrm < A.max_norm;
+            // squared norm against the squared limit: no square root per iteration
+            const double nrm2 = convergence_norm2_dev<M>(dp, size);
+            const bool conv = ok && nrm2 < max_norm2;
             if (ok && A.hist && it <= A.hist_cap && lane == 0) {
+                const double nrm = sqrt(nrm2);
                 const double znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
                 double* hh = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
                 hh[0] = (double)it;
@@ -1065,7 +1117,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             if (ok && !conv && it < A.max_it) {
 #pragma unroll
                 for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
-                nx = make_plan<M>(pn, ext, xc, yc, fxc, fyc, W, A);
+                nx = make_plan<M, S>(pn, ext, xc, yc, fxc, fyc, W, A);
             }
             if (lane == 0) {
 #pragma unroll
@@ -1074,7 +1126,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 sCtl[1] = conv ? 1 : 0;
                 sG[0] = gm;
                 sG[1] = dg;
-                sG[2] = nrm;
+                sG[2] = nrm2;
                 sPlan = nx;
             }
         }
@@ -1108,7 +1160,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     if (it_done) {
         g_m = uni(sG[0]);
         delta_g = uni(sG[1]);
-        last_norm = uni(sG[2]);
+        last_norm = uni(sqrt(sG[2]));
     }
 
     // ---- final ZNSSD (subset.rs:1190-1193) and the diagnostic's projections
@@ -1129,7 +1181,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         const Window W = plan_window(pl);
         const bool fast = it_done && pl.fast;
         const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
-        const int cb = (by - W.cy_lo) * W.ww + (bx - W.cx_lo);
+        const int LS = S > 0 ? S : W.ls;
+        const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
 #pragma unroll 1
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
@@ -1141,7 +1194,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                     warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
                     const float rx = rintf(ox), ry = rintf(oy);
                     float cc[36];
-                    gather6(win + cb + (int)fmaf(ry, (float)W.ww, rx), W.ww, cc);
+                    gather_block<S>(win, cb + (int)fmaf(ry, (float)LS, rx), W.ls, cc);
                     float a[6], b[6];
                     basis_f32_s(oy - ry, a);
                     basis_f32_s(ox - rx, b);

@@ -190,6 +190,28 @@ __device__ bool compose_icgn_dev(const double* p, const double* dp, double* out,
     return true;
 }
 
+// Square of convergence_norm (subset.rs:622-649), for comparisons.
+template <int M>
+__device__ inline double convergence_norm2_dev(const double* d, double size) {
+    double s = d[0] * d[0] + d[1] * d[1];
+    if (M >= 6) {
+#pragma unroll
+        for (int k = 2; k < 6; ++k) {
+            const double t = d[k] * size;
+            s = fma(t, t, s);
+        }
+    }
+    if (M == 12) {
+        const double h = 0.5 * size * size;
+#pragma unroll
+        for (int k = 6; k < 12; ++k) {
+            const double t = d[k] * h;
+            s = fma(t, t, s);
+        }
+    }
+    return s;
+}
+
 // convergence_norm (subset.rs:622-649)
 template <int M>
 __device__ inline double convergence_norm_dev(const double* d, double size) {

@@ -135,20 +135,29 @@ int main(int argc, char** argv) {
     A.tol = 0.75;
     A.max_norm = 1e-5;
     const int wside = (ext[1] - ext[0] + 1) + 2 * slack + 12;
-    A.win_cap = ((wside + 3) & ~3) * wside;
+    const int wrow = (wside + 3) & ~3;
+    const bool s64 = wrow <= 64;
+    A.win_cap = (s64 ? 64 : wrow) * wside;
     A.win_slack = slack;
     const size_t lds = sizeof(float) * A.win_cap + 2 * sizeof(float) * npx;
     // P = 12 pixels per thread: one wavefront holds 768 pixels
     auto run = [&](auto mtag) {
         constexpr int MM = decltype(mtag)::value;
-        if (wps == 1)
-            simt::launch(n, 64, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 1>(A); });
-        else if (wps == 2)
-            simt::launch(n, 128, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 2>(A); });
-        else if (wps == 4)
-            simt::launch(n, 256, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 4>(A); });
+        auto go = [&](auto stag) {
+            constexpr int SS = decltype(stag)::value;
+            if (wps == 1)
+                simt::launch(n, 64, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 1, SS>(A); });
+            else if (wps == 2)
+                simt::launch(n, 128, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 2, SS>(A); });
+            else if (wps == 4)
+                simt::launch(n, 256, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 4, SS>(A); });
+            else
+                std::abort();
+        };
+        if (s64)
+            go(std::integral_constant<int, 64>{});
         else
-            std::abort();
+            go(std::integral_constant<int, 0>{});
     };
     if (M == 2)
         run(std::integral_constant<int, 2>{});
