@@ -1,6 +1,8 @@
 // C ABI of the engine (include/geodic.h): engine, images, batches.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <string>
 
@@ -234,6 +236,7 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_tpl_index, sizeof(int32_t) * n);
     dev_free(e, b->d_tpl_start, sizeof(int64_t) * (b->n_tpl + 1));
     dev_free(e, b->d_tpl_xy, sizeof(int32_t) * (b->h_tpl_start.empty() ? 0 : b->h_tpl_start.back()));
+    dev_free(e, b->d_tpl_xy_eng, sizeof(int32_t) * (b->h_tpl_start.empty() ? 0 : b->h_tpl_start.back()));
     dev_free(e, b->d_tpl_ext, sizeof(int32_t) * 4 * b->n_tpl);
     dev_free(e, b->d_scr_off, sizeof(int64_t) * n);
     dev_free(e, b->d_p0, sizeof(double) * 12 * n);
@@ -246,6 +249,52 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_order_perm, sizeof(int32_t) * n);
     dev_free(e, b->d_queue, sizeof(int));
 }
+
+namespace {
+
+// Pixel order of a template for the batched engine.  A wavefront gathers
+// 64 pixels' coefficient blocks at once from an LDS window whose rows are 72
+// floats apart, so pixel (dx, dy) starts in bank (dx + 8 dy) mod 64 (72 = 8
+// mod 64).  Whole 8x8 tiles of the template give 64 distinct banks (and
+// keep them under a near-uniform warp); the pixels of partial tiles are
+// dealt into groups of 64 taking one pixel per bank, fullest banks first.
+// Sums over the subset do not depend on the order beyond rounding.
+void engine_pixel_order(const int32_t* in, int64_t n, int32_t* out) {
+    auto ux = [](int32_t v) { return (int)(int16_t)(v & 0xffff); };
+    auto uy = [](int32_t v) { return (int)(int16_t)((uint32_t)v >> 16); };
+    if (n == 0) return;
+    int x0 = ux(in[0]), y0 = uy(in[0]);
+    for (int64_t i = 1; i < n; ++i) {
+        x0 = std::min(x0, ux(in[i]));
+        y0 = std::min(y0, uy(in[i]));
+    }
+    std::map<std::pair<int, int>, std::vector<int32_t>> tiles;
+    for (int64_t i = 0; i < n; ++i) tiles[{(uy(in[i]) - y0) >> 3, (ux(in[i]) - x0) >> 3}].push_back(in[i]);
+    int64_t k = 0;
+    std::vector<std::vector<int32_t>> bank(64);
+    for (auto& kv : tiles) {
+        if (kv.second.size() == 64) {
+            for (int32_t v : kv.second) out[k++] = v;
+        } else {
+            for (int32_t v : kv.second) bank[(unsigned)(ux(v) + 8 * uy(v)) & 63].push_back(v);
+        }
+    }
+    for (auto& b : bank) std::reverse(b.begin(), b.end());
+    while (k < n) {
+        std::vector<int> ord(64);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return bank[a].size() > bank[c].size(); });
+        int taken = 0;
+        for (int b : ord)
+            if (!bank[b].empty() && taken < 64) {
+                out[k++] = bank[b].back();
+                bank[b].pop_back();
+                ++taken;
+            }
+    }
+}
+
+}  // namespace
 
 int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, const int32_t* orders,
                      const int32_t* tpl_index, int32_t n_templates, const int64_t* tpl_start, const int16_t* tpl_xy,
@@ -306,7 +355,10 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
         const int t = tpl_index[s];
         b->total_px += tpl_start[t + 1] - tpl_start[t];
     }
-    // largest subsets first: the persistent grid drains the tail evenly
+    b->h_coords.assign(coords, coords + 2 * n_subsets);
+    b->h_ext = ext;
+    // provisional dispatch order (largest subsets first); refined for the
+    // image shape at the first solve
     std::vector<int32_t> perm(n_subsets);
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t c) {
@@ -323,6 +375,7 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     A((void**)&b->d_tpl_index, sizeof(int32_t) * n_subsets);
     A((void**)&b->d_tpl_start, sizeof(int64_t) * (n_templates + 1));
     A((void**)&b->d_tpl_xy, sizeof(int32_t) * n_off);
+    A((void**)&b->d_tpl_xy_eng, sizeof(int32_t) * n_off);
     A((void**)&b->d_tpl_ext, sizeof(int32_t) * 4 * n_templates);
     A((void**)&b->d_scr_off, sizeof(int64_t) * n_subsets);
     A((void**)&b->d_slab_off, sizeof(int64_t) * n_subsets);
@@ -342,6 +395,10 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     H2D(b->d_tpl_index, tpl_index, sizeof(int32_t) * n_subsets);
     H2D(b->d_tpl_start, tpl_start, sizeof(int64_t) * (n_templates + 1));
     H2D(b->d_tpl_xy, packed.data(), sizeof(int32_t) * n_off);
+    std::vector<int32_t> packed_eng(n_off);
+    for (int t = 0; t < n_templates; ++t) engine_pixel_order(packed.data() + tpl_start[t], tpl_start[t + 1] - tpl_start[t],
+                                                            packed_eng.data() + tpl_start[t]);
+    H2D(b->d_tpl_xy_eng, packed_eng.data(), sizeof(int32_t) * n_off);
     H2D(b->d_tpl_ext, ext.data(), sizeof(int32_t) * 4 * n_templates);
     H2D(b->d_order_perm, perm.data(), sizeof(int32_t) * n_subsets);
     if (!st && hipMemsetAsync(b->d_p0, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
@@ -381,6 +438,43 @@ int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_subset) {
     return DIC_OK;
 }
 
+namespace {
+
+// Dispatch order for an image of w x h: largest subsets first, then the
+// subsets whose template box lies nearest the image edge (clamped pixels
+// make them the slow ones: they converge late or run to the iteration cap),
+// so the work that decides the launch's tail starts early.
+int refresh_order(Batch* b, int64_t w, int64_t h) {
+    if (b->perm_w == w && b->perm_h == h) return DIC_OK;
+    const int64_t n = b->n;
+    std::vector<int64_t> npx(n);
+    std::vector<double> edge(n);
+    for (int64_t s = 0; s < n; ++s) {
+        const int t = b->h_tpl_index[s];
+        npx[s] = b->h_tpl_start[t + 1] - b->h_tpl_start[t];
+        const double xc = b->h_coords[2 * s], yc = b->h_coords[2 * s + 1];
+        const int32_t* e = &b->h_ext[4 * (size_t)t];
+        const double d = std::min(std::min(xc + e[0], (double)(w - 1) - (xc + e[1])),
+                                  std::min(yc + e[2], (double)(h - 1) - (yc + e[3])));
+        edge[s] = std::isfinite(d) ? d : -1e300;  // non-finite centres first
+    }
+    b->h_perm.resize(n);
+    std::iota(b->h_perm.begin(), b->h_perm.end(), 0);
+    std::stable_sort(b->h_perm.begin(), b->h_perm.end(), [&](int32_t a, int32_t c) {
+        if (npx[a] != npx[c]) return npx[a] > npx[c];
+        return edge[a] < edge[c];
+    });
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipMemcpyAsync(b->d_order_perm, b->h_perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                               e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    b->perm_w = w;
+    b->perm_h = h;
+    return DIC_OK;
+}
+
+}  // namespace
+
 int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const dic_solve_params* prm) {
     CHECK_ARG(db && f && g && prm, "dic_batch_solve: NULL argument");
     Batch* b = &db->b;
@@ -408,7 +502,8 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
         }
         DIC_HIP_TRY(hipMemsetAsync(b->d_hist, 0, sizeof(double) * 4 * b->n * b->hist_cap, e->stream));
     }
-    int st;
+    int st = refresh_order(b, f->im.w, f->im.h);
+    if (st) return st;
     if (prm->precision == DIC_PREC_PLAIN_F64) {
         // scratch sizing and offsets are host work done before the timed launch
         DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));

@@ -299,6 +299,20 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
+// A template offset pair as stored in LDS: two half-precision numbers
+// (offsets are integers far inside the exact range of FP16), so the FP32
+// arithmetic that uses them can take them as mixed-precision operands.
+#ifdef GEODIC_SIMT
+typedef int16_t offh_t;
+#else
+typedef _Float16 offh_t;
+#endif
+struct OffXY {
+    offh_t x, y;
+};
+
+__device__ __forceinline__ float2 off_xy(const OffXY o) { return make_float2((float)o.x, (float)o.y); }
+
 __device__ __forceinline__ int unpack_dx(int32_t v) { return (int)(int16_t)(v & 0xffff); }
 __device__ __forceinline__ int unpack_dy(int32_t v) { return (int)(int16_t)((uint32_t)v >> 16); }
 

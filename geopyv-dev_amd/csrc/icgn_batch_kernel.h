@@ -487,7 +487,7 @@ __device__ __forceinline__ void sdi32(float gx, float gy, float dx, float dy, fl
 
 // Hessian rows [A0, A1) from the register-resident gradients.
 template <int M, int P, int WPS, int A0, int A1>
-__device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr, const float2* sXY, int n, int tid,
+__device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr, const OffXY* sXY, int n, int tid,
                                              int lane, int wave, double* sH, double* sRed) {
     constexpr int NT = 64 * WPS;
     constexpr int NE = (A1 - A0) * M - ((A1 - 1) * A1 - (A0 - 1) * A0) / 2;
@@ -498,7 +498,7 @@ __device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr,
     for (int k = 0; k < P; ++k) {
         const int i = tid + NT * k;
         if (i < n) {
-            const float2 d = sXY[i];
+            const float2 d = off_xy(sXY[i]);
             float sd[M];
             sdi32<M>(gxr[k], gyr[k], d.x, d.y, sd);
             int q = 0;
@@ -738,7 +738,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     constexpr int V2 = 4 + 4 * K;                      // second setup pass
     GEODIC_DYN_LDS(float, lds);
     float* win = lds;
-    float2* sXY = (float2*)(lds + A.win_cap);
+    OffXY* sXY = (OffXY*)(lds + A.win_cap);
     __shared__ double sH[M * M];
     __shared__ double sL[NL];
     __shared__ double sRd[M];
@@ -797,7 +797,10 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     }
     for (int i = tid; i < n; i += NT) {
         const int32_t o = offs[i];
-        sXY[i] = make_float2((float)unpack_dx(o), (float)unpack_dy(o));
+        OffXY d;
+        d.x = (offh_t)unpack_dx(o);
+        d.y = (offh_t)unpack_dy(o);
+        sXY[i] = d;
     }
 
     float am[P], gxr[P], gyr[P];
@@ -841,7 +844,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             const int i = tid + NT * k;
             float v = 0.f, gx = 0.f, gy = 0.f;
             if (i < n) {
-                const float2 d = sXY[i];
+                const float2 d = off_xy(sXY[i]);
                 if (fast) {
                     float cc[36];
                     gather_block<S>(win, cb + (int)fmaf(d.y, wwf, d.x), W.ls, cc);
@@ -935,7 +938,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
-                const float2 d = sXY[i];
+                const float2 d = off_xy(sXY[i]);
                 const float a = am[k] - d32;
                 am[k] = a;
                 v2[0] += a;
@@ -1026,7 +1029,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             for (int k = 0; k < P; ++k) {
                 const int i = tid + NT * k;
                 if (i < n) {
-                    const float2 d = sXY[i];
+                    const float2 d = off_xy(sXY[i]);
                     float ox, oy;
                     warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
                     const float rx = rintf(ox), ry = rintf(oy);
@@ -1044,7 +1047,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             for (int k = 0; k < P; ++k) {
                 const int i = tid + NT * k;
                 if (i < n) {
-                    const float2 d = sXY[i];
+                    const float2 d = off_xy(sXY[i]);
                     float ox, oy;
                     warp_pos<M>(pl.q, pl.c0x, pl.c0y, d.x, d.y, ox, oy);
                     int cx, cy;
@@ -1187,7 +1190,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
-                const float2 d = sXY[i];
+                const float2 d = off_xy(sXY[i]);
                 float gp = 0.f;  // g - f_m
                 if (fast) {
                     float ox, oy;

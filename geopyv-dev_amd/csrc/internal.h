@@ -65,7 +65,8 @@ struct Batch {
     int32_t* d_orders = nullptr;    // n
     int32_t* d_tpl_index = nullptr; // n
     int64_t* d_tpl_start = nullptr; // n_tpl + 1
-    int32_t* d_tpl_xy = nullptr;    // packed (dx & 0xffff) | (dy << 16)
+    int32_t* d_tpl_xy = nullptr;    // packed (dx & 0xffff) | (dy << 16), reference scan order
+    int32_t* d_tpl_xy_eng = nullptr;  // the same pixels in LDS-bank-friendly order (batched engine)
     int32_t* d_tpl_ext = nullptr;   // n_tpl x 4: dxmin, dxmax, dymin, dymax
     int32_t max_side = 0;           // largest template box side
     int32_t win_slack = 3;          // deformation slack of the LDS window (pixels per side)
@@ -82,7 +83,11 @@ struct Batch {
     size_t slab_floats = 0;
     int64_t* d_slab_off = nullptr;  // n: first pixel of each subset in the slab
     bool slab_off_ready = false;
-    int32_t* d_order_perm = nullptr;  // subsets sorted by template size (descending)
+    int32_t* d_order_perm = nullptr;  // dispatch order: template size descending, then nearest the image edge first
+    std::vector<double> h_coords;       // host copies for the dispatch order
+    std::vector<int32_t> h_ext;
+    std::vector<int32_t> h_perm;
+    int64_t perm_w = -1, perm_h = -1;   // image shape the order was made for
     int32_t* d_queue = nullptr;       // work-queue counter
     // host copies
     std::vector<int64_t> h_tpl_start;

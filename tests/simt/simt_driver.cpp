@@ -134,12 +134,12 @@ int main(int argc, char** argv) {
     A.max_it = max_it;
     A.tol = 0.75;
     A.max_norm = 1e-5;
-    const int wside = (ext[1] - ext[0] + 1) + 2 * slack + 12;
-    const int wrow = (wside + 3) & ~3;
-    const bool s64 = wrow <= 64;
-    A.win_cap = (s64 ? 64 : wrow) * wside;
+    const int wside = (ext[1] - ext[0] + 1) + 2 * slack + 8;
+    const int wrow = (wside + 3 + 3) & ~3;
+    const bool s72 = wrow <= 72;
+    A.win_cap = (s72 ? 72 : wrow) * wside;
     A.win_slack = slack;
-    const size_t lds = sizeof(float) * A.win_cap + 2 * sizeof(float) * npx;
+    const size_t lds = sizeof(float) * A.win_cap + 4 * (size_t)npx;
     // P = 12 pixels per thread: one wavefront holds 768 pixels
     auto run = [&](auto mtag) {
         constexpr int MM = decltype(mtag)::value;
@@ -154,8 +154,8 @@ int main(int argc, char** argv) {
             else
                 std::abort();
         };
-        if (s64)
-            go(std::integral_constant<int, 64>{});
+        if (s72)
+            go(std::integral_constant<int, 72>{});
         else
             go(std::integral_constant<int, 0>{});
     };
