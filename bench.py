@@ -193,7 +193,7 @@ def main():
         "converged_fraction": round(float(np.mean(res32["converged"])), 5),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "bytes_per_subset": round(bytes_per_subset, 1), "kernel": "icgn_batch_kernel<6>"},
+                     "bytes_per_subset": round(bytes_per_subset, 1), "kernel": "icgn_batch_kernel (order 1)"},
         "error_vs_truth_px": {"f32_converged": float(np.abs(res32["p"][:, :2] - truth[:, :2])[res32["converged"] == 1].max()),
                               "fp64_plain_converged": float(np.abs(res64["p"][:, :2] - truth[:, :2])[res64["converged"] == 1].max()),
                               "not_converged": int(np.sum(res64["converged"] == 0))},

@@ -440,15 +440,20 @@ int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_subset) {
 
 namespace {
 
-// Dispatch order for an image of w x h: largest subsets first, then the
-// subsets whose template box lies nearest the image edge (clamped pixels
-// make them the slow ones: they converge late or run to the iteration cap),
-// so the work that decides the launch's tail starts early.
+// Dispatch order for an image of w x h.  Largest subsets first.  Within
+// a size, the subsets (in their given, usually raster, order) are cut into
+// eight contiguous bands and dealt out so that dispatch slot b takes band
+// b mod 8: the hardware sends workgroup b to XCD b mod 8, so each XCD works
+// through one band and neighbouring subsets share its L2 (an eighth of the
+// HBM traffic of raster order).  Inside a band the subsets go in order of
+// the distance of their template box from the image edge, nearest first:
+// clamped pixels make the edge subsets the slow ones (late convergence or
+// the iteration cap), and started late they would decide the launch's tail.
 int refresh_order(Batch* b, int64_t w, int64_t h) {
     if (b->perm_w == w && b->perm_h == h) return DIC_OK;
     const int64_t n = b->n;
     std::vector<int64_t> npx(n);
-    std::vector<double> edge(n);
+    std::vector<double> dist(n);
     for (int64_t s = 0; s < n; ++s) {
         const int t = b->h_tpl_index[s];
         npx[s] = b->h_tpl_start[t + 1] - b->h_tpl_start[t];
@@ -456,14 +461,32 @@ int refresh_order(Batch* b, int64_t w, int64_t h) {
         const int32_t* e = &b->h_ext[4 * (size_t)t];
         const double d = std::min(std::min(xc + e[0], (double)(w - 1) - (xc + e[1])),
                                   std::min(yc + e[2], (double)(h - 1) - (yc + e[3])));
-        edge[s] = std::isfinite(d) ? d : -1e300;  // non-finite centres first
+        dist[s] = std::isfinite(d) ? d : -1e300;  // non-finite centres first
     }
-    b->h_perm.resize(n);
-    std::iota(b->h_perm.begin(), b->h_perm.end(), 0);
-    std::stable_sort(b->h_perm.begin(), b->h_perm.end(), [&](int32_t a, int32_t c) {
-        if (npx[a] != npx[c]) return npx[a] > npx[c];
-        return edge[a] < edge[c];
-    });
+    std::vector<int32_t> by_size(n);
+    std::iota(by_size.begin(), by_size.end(), 0);
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int32_t a, int32_t c) { return npx[a] > npx[c]; });
+    b->h_perm.clear();
+    b->h_perm.reserve(n);
+    for (int64_t g0 = 0; g0 < n;) {
+        int64_t g1 = g0;
+        while (g1 < n && npx[by_size[g1]] == npx[by_size[g0]]) ++g1;
+        const int64_t len = g1 - g0;
+        std::vector<std::vector<int32_t>> band(8);
+        for (int x = 0; x < 8; ++x) {
+            const int64_t a = g0 + len * x / 8, c = g0 + len * (x + 1) / 8;
+            for (int64_t i = a; i < c; ++i) band[x].push_back(by_size[i]);
+            std::stable_sort(band[x].begin(), band[x].end(), [&](int32_t u, int32_t q) { return dist[u] < dist[q]; });
+        }
+        for (size_t j = 0; j < band[7].size() + 1; ++j)
+            for (int x = 0; x < 8; ++x)
+                if (j < band[x].size()) b->h_perm.push_back(band[x][j]);
+        g0 = g1;
+    }
+    if ((int64_t)b->h_perm.size() != n) {
+        set_error("dispatch order: internal error");
+        return DIC_ERR_DEVICE;
+    }
     Engine* e = b->eng;
     DIC_HIP_TRY(hipMemcpyAsync(b->d_order_perm, b->h_perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
                                e->stream));
