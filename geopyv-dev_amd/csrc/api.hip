@@ -73,10 +73,17 @@ int dic_engine_create(int device, dic_engine** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         e->num_cus = prop.multiProcessorCount;
-    // default priority stream (mixed stream priorities are avoided)
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e->ev_pre0) != hipSuccess || hipEventCreate(&e->ev_pre1) != hipSuccess ||
-        hipEventCreate(&e->ev_sol0) != hipSuccess || hipEventCreate(&e->ev_sol1) != hipSuccess) {
+    // default-priority streams only (mixed stream priorities are avoided):
+    // the engine's stream plus three that run size classes of a batch
+    // alongside it
+    bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&e->ev_pre0) == hipSuccess && hipEventCreate(&e->ev_pre1) == hipSuccess &&
+              hipEventCreate(&e->ev_sol0) == hipSuccess && hipEventCreate(&e->ev_sol1) == hipSuccess &&
+              hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; ok && k < Engine::kAux; ++k)
+        ok = hipStreamCreateWithFlags(&e->aux[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&e->ev_join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         set_error("dic_engine_create: stream/event creation failed");
         delete d;
         return DIC_ERR_DEVICE;
@@ -97,6 +104,14 @@ void dic_engine_destroy(dic_engine* d) {
     (void)hipEventDestroy(e->ev_pre1);
     (void)hipEventDestroy(e->ev_sol0);
     (void)hipEventDestroy(e->ev_sol1);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    for (int k = 0; k < Engine::kAux; ++k) {
+        if (e->aux[k]) {
+            (void)hipStreamSynchronize(e->aux[k]);
+            (void)hipStreamDestroy(e->aux[k]);
+        }
+        if (e->ev_join[k]) (void)hipEventDestroy(e->ev_join[k]);
+    }
     (void)hipStreamDestroy(e->stream);
     delete d;
 }
@@ -244,10 +259,7 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_res, sizeof(dic_subset_result) * n);
     dev_free(e, b->d_hist, sizeof(double) * 4 * n * b->hist_cap);
     dev_free(e, b->d_scratch, sizeof(double) * b->scratch_doubles);
-    dev_free(e, b->d_slab, sizeof(float) * b->slab_floats);
-    dev_free(e, b->d_slab_off, sizeof(int64_t) * n);
     dev_free(e, b->d_order_perm, sizeof(int32_t) * n);
-    dev_free(e, b->d_queue, sizeof(int));
 }
 
 namespace {
@@ -378,12 +390,10 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     A((void**)&b->d_tpl_xy_eng, sizeof(int32_t) * n_off);
     A((void**)&b->d_tpl_ext, sizeof(int32_t) * 4 * n_templates);
     A((void**)&b->d_scr_off, sizeof(int64_t) * n_subsets);
-    A((void**)&b->d_slab_off, sizeof(int64_t) * n_subsets);
     A((void**)&b->d_p0, sizeof(double) * 12 * n_subsets);
     A((void**)&b->d_p, sizeof(double) * 12 * n_subsets);
     A((void**)&b->d_res, sizeof(dic_subset_result) * n_subsets);
     A((void**)&b->d_order_perm, sizeof(int32_t) * n_subsets);
-    A((void**)&b->d_queue, sizeof(int));
     auto H2D = [&](void* dst, const void* src, size_t bytes) {
         if (!st && bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess) {
             set_error("dic_batch_create: upload failed");

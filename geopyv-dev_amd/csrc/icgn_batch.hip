@@ -11,48 +11,75 @@ using batch::icgn_batch_kernel;
 using batch::kRedSlots;
 
 template <int M, int P, int WPS, int S>
-int launch_mpw(Batch* b, BatchArgs& A, int blocks, size_t lds) {
-    Engine* e = b->eng;
+int launch_mpw(const BatchArgs& A, int blocks, size_t lds, hipStream_t stream) {
     DIC_HIP_TRY(hipFuncSetAttribute((const void*)icgn_batch_kernel<M, P, WPS, S>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    icgn_batch_kernel<M, P, WPS, S><<<blocks, 64 * WPS, lds, e->stream>>>(A);
+    icgn_batch_kernel<M, P, WPS, S><<<blocks, 64 * WPS, lds, stream>>>(A);
     DIC_HIP_TRY(hipGetLastError());
     return DIC_OK;
 }
 
-// LDS row stride: 72 floats when every window of the batch fits it (rows
+// LDS row stride: 72 floats when every window of the class fits it (rows
 // then sit at immediate offsets, and 72 = 8 mod 64 spreads an 8x8 tile of
 // pixels over all 64 banks), else the window's own width.
 template <int M, int WPS>
-int launch_ms(Batch* b, BatchArgs& A, int blocks, size_t lds, bool s72) {
-    return s72 ? launch_mpw<M, 12, WPS, 72>(b, A, blocks, lds) : launch_mpw<M, 12, WPS, 0>(b, A, blocks, lds);
+int launch_ms(const BatchArgs& A, int blocks, size_t lds, bool s72, hipStream_t stream) {
+    return s72 ? launch_mpw<M, 12, WPS, 72>(A, blocks, lds, stream)
+               : launch_mpw<M, 12, WPS, 0>(A, blocks, lds, stream);
 }
 
-// Enough wavefronts per subset to hold the largest subset at 12 pixels per
-// thread (the footprint that stays in registers without spilling).
 template <int M>
-int launch_m(Batch* b, BatchArgs& A, int blocks, size_t lds, bool s72) {
-    const int n = b->max_n;
-    int wps = b->waves_per_subset;
-    if (wps <= 0) wps = n <= 64 * 12 ? 1 : (n <= 128 * 12 ? 2 : (n <= 256 * 12 ? 4 : 8));
-    if (n > 64 * 12 * wps) {
-        set_error("invalid input: subset larger than the wavefronts per subset can hold");
-        return DIC_ERR_INVALID_INPUT;
-    }
+int launch_m(const BatchArgs& A, int wps, int blocks, size_t lds, bool s72, hipStream_t stream) {
     switch (wps) {
-        case 1: return launch_ms<M, 1>(b, A, blocks, lds, s72);
-        case 2: return launch_ms<M, 2>(b, A, blocks, lds, s72);
-        case 4: return launch_ms<M, 4>(b, A, blocks, lds, s72);
-        case 8: return launch_ms<M, 8>(b, A, blocks, lds, s72);
+        case 1: return launch_ms<M, 1>(A, blocks, lds, s72, stream);
+        case 2: return launch_ms<M, 2>(A, blocks, lds, s72, stream);
+        case 4: return launch_ms<M, 4>(A, blocks, lds, s72, stream);
+        case 8: return launch_ms<M, 8>(A, blocks, lds, s72, stream);
         default: break;
     }
     set_error("invalid input: waves per subset must be 1, 2, 4 or 8");
     return DIC_ERR_INVALID_INPUT;
 }
 
+// Fewest wavefronts that hold n pixels at 12 per thread (the footprint that
+// stays in registers without spilling).
+int min_wps(int64_t n) { return n <= 64 * 12 ? 1 : (n <= 128 * 12 ? 2 : (n <= 256 * 12 ? 4 : 8)); }
+
 }  // namespace
 
+// Size classes of a batch in dispatch order (the order sorts by size, so
+// each class is a contiguous run): subsets that need the same number of
+// wavefronts launch together; an explicit schedule makes one class.
+std::vector<SizeClass> size_classes(const Batch* b) {
+    std::vector<SizeClass> out;
+    auto npx = [&](int64_t s) {
+        const int t = b->h_tpl_index[s];
+        return b->h_tpl_start[t + 1] - b->h_tpl_start[t];
+    };
+    auto side = [&](int64_t s) {
+        const int32_t* e = &b->h_ext[4 * (size_t)b->h_tpl_index[s]];
+        return std::max(e[1] - e[0] + 1, e[3] - e[2] + 1);
+    };
+    for (int64_t i = 0; i < b->n;) {
+        const int w = b->waves_per_subset > 0 ? b->waves_per_subset : min_wps(npx(b->h_perm[i]));
+        SizeClass c;
+        c.start = i;
+        c.wps = w;
+        c.max_n = 0;
+        c.max_side = 0;
+        while (i < b->n && (b->waves_per_subset > 0 || min_wps(npx(b->h_perm[i])) == w)) {
+            c.max_n = std::max<int64_t>(c.max_n, npx(b->h_perm[i]));
+            c.max_side = std::max(c.max_side, side(b->h_perm[i]));
+            ++i;
+        }
+        c.count = i - c.start;
+        out.push_back(c);
+    }
+    return out;
+}
+
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+    Engine* e = b->eng;
     BatchArgs A;
     A.fC = f->coef32;
     A.gC = g->coef32;
@@ -62,13 +89,11 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.h = (int)f->h;
     A.wmax = (double)f->w - 1e-9;
     A.hmax = (double)f->h - 1e-9;
-    A.n_sub = b->n;
     A.coords = b->d_coords;
     A.tpl_index = b->d_tpl_index;
     A.tpl_start = b->d_tpl_start;
     A.tpl_xy = b->d_tpl_xy_eng;
     A.tpl_ext = b->d_tpl_ext;
-    A.perm = b->d_order_perm;
     A.p0 = b->d_p0;
     A.p_out = b->d_p;
     A.res = b->d_res;
@@ -77,22 +102,48 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.max_it = prm->max_iterations;
     A.tol = prm->tolerance;
     A.max_norm = prm->max_norm;
-    // LDS: the coefficient window (template box, block taps, 16-byte
-    // rounding and the restaging slack, with room for moderate strain)
-    // followed by the template offsets as FP32 pairs
-    // LDS: the coefficient window (template box, slack, taps, cell rounding
-    // and a little strain), then the template offsets as FP16 pairs
-    const int side = b->max_side + 2 * b->win_slack + 8;
-    const int row = (side + 3 + 3) & ~3;  // plus the 16-byte rounding of the first column
-    const bool s72 = row <= 72;
-    A.win_cap = (s72 ? 72 : row) * side;
     A.win_slack = b->win_slack;
-    const size_t lds = sizeof(float) * (size_t)A.win_cap + 4 * (size_t)b->max_n;
-    const int blocks = (int)b->n;
-    const int order = b->max_order;
-    if (order == 0) return launch_m<2>(b, A, blocks, lds, s72);
-    if (order == 1) return launch_m<6>(b, A, blocks, lds, s72);
-    return launch_m<12>(b, A, blocks, lds, s72);
+    const std::vector<SizeClass> cls = size_classes(b);
+    for (const SizeClass& c : cls)
+        if (c.max_n > 64 * 12 * c.wps) {
+            set_error("invalid input: subset larger than the wavefronts per subset can hold");
+            return DIC_ERR_INVALID_INPUT;
+        }
+    // the first class runs on the engine's stream, the others on the
+    // auxiliary streams, forked from and joined back into it
+    const int nstreams = std::min<int>((int)cls.size(), 1 + Engine::kAux);
+    if (nstreams > 1) {
+        DIC_HIP_TRY(hipEventRecord(e->ev_fork, e->stream));
+        for (int k = 0; k + 1 < nstreams; ++k) DIC_HIP_TRY(hipStreamWaitEvent(e->aux[k], e->ev_fork, 0));
+    }
+    for (size_t k = 0; k < cls.size(); ++k) {
+        const SizeClass& c = cls[k];
+        BatchArgs Ac = A;
+        Ac.perm = b->d_order_perm + c.start;
+        Ac.n_sub = c.count;
+        // LDS: the coefficient window (template box, slack, taps, cell
+        // rounding and a little strain), then the template offsets as FP16
+        // pairs
+        const int side = c.max_side + 2 * b->win_slack + 8;
+        const int row = (side + 3 + 3) & ~3;  // plus the 16-byte rounding of the first column
+        const bool s72 = row <= 72;
+        Ac.win_cap = (s72 ? 72 : row) * side;
+        const size_t lds = sizeof(float) * (size_t)Ac.win_cap + 4 * (size_t)c.max_n;
+        hipStream_t st = (k % nstreams == 0) ? e->stream : e->aux[k % nstreams - 1];
+        int r;
+        if (b->max_order == 0)
+            r = launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st);
+        else if (b->max_order == 1)
+            r = launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st);
+        else
+            r = launch_m<12>(Ac, c.wps, (int)c.count, lds, s72, st);
+        if (r) return r;
+    }
+    for (int k = 0; k + 1 < nstreams; ++k) {
+        DIC_HIP_TRY(hipEventRecord(e->ev_join[k], e->aux[k]));
+        DIC_HIP_TRY(hipStreamWaitEvent(e->stream, e->ev_join[k], 0));
+    }
+    return DIC_OK;
 }
 
 }  // namespace geodic

@@ -29,6 +29,9 @@ struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_pre0 = nullptr, ev_pre1 = nullptr, ev_sol0 = nullptr, ev_sol1 = nullptr;
+    static constexpr int kAux = 3;
+    hipStream_t aux[kAux] = {nullptr, nullptr, nullptr};  // size classes of a batch run here too
+    hipEvent_t ev_fork = nullptr, ev_join[kAux] = {nullptr, nullptr, nullptr};
     bool pre_recorded = false, sol_recorded = false;
     int64_t bytes_in_use = 0;
     int num_cus = 256;
@@ -79,16 +82,11 @@ struct Batch {
     int32_t hist_cap = 0;
     double* d_scratch = nullptr;    // plain path per-pixel arrays
     size_t scratch_doubles = 0;
-    float* d_slab = nullptr;        // batched engine per-pixel slab (float4 per pixel)
-    size_t slab_floats = 0;
-    int64_t* d_slab_off = nullptr;  // n: first pixel of each subset in the slab
-    bool slab_off_ready = false;
     int32_t* d_order_perm = nullptr;  // dispatch order: template size descending, then nearest the image edge first
     std::vector<double> h_coords;       // host copies for the dispatch order
     std::vector<int32_t> h_ext;
     std::vector<int32_t> h_perm;
     int64_t perm_w = -1, perm_h = -1;   // image shape the order was made for
-    int32_t* d_queue = nullptr;       // work-queue counter
     // host copies
     std::vector<int64_t> h_tpl_start;
     std::vector<int32_t> h_tpl_index;
@@ -104,6 +102,13 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem)
 // icgn_plain.hip
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 // icgn_batch.hip
+struct SizeClass {
+    int64_t start, count;  // run of the dispatch order
+    int wps;               // wavefronts per subset
+    int64_t max_n;
+    int max_side;
+};
+std::vector<SizeClass> size_classes(const Batch* b);
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 
 // device memory accounting

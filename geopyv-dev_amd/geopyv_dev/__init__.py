@@ -498,10 +498,20 @@ def solve_subsets(f_img, g_img, coords, local_mask, p0=None, subset_order=1, pre
                   max_norm=1e-5, max_iterations=50, tolerance=0.75, global_mask=None):
     """Solve many subsets in one launch (Subset::new + solve per subset).
 
-    precision "f64" is the plain FP64 drop-in, "f32" the batched engine.
-    Returns a structured array of results (fields as dic_subset_result)."""
+    local_mask is one Mask for every subset or a sequence of Masks, one per
+    subset (uneven radii).  precision "f64" is the plain FP64 drop-in, "f32"
+    the batched engine.  Returns a structured array of results (fields as
+    dic_subset_result)."""
     coords = np.asarray(coords, dtype=np.float64).reshape(-1, 2)
-    if global_mask is not None:
+    if isinstance(local_mask, (list, tuple)):
+        if len(local_mask) != len(coords):
+            raise ValueError("one local mask per subset is required")
+        if global_mask is not None:
+            lists = [_native.mask_update(m.shape, m.size, c, global_mask.binary) for m, c in zip(local_mask, coords)]
+        else:
+            lists = [m.coords for m in local_mask]
+        index, start, xy = templates_from_masks(lists)
+    elif global_mask is not None:
         lists = [_native.mask_update(local_mask.shape, local_mask.size, c, global_mask.binary) for c in coords]
         index, start, xy = templates_from_masks(lists)
     else:

@@ -179,3 +179,123 @@ def uniform_mesh_pair(seed=1, n_side=200, shape=(2048, 2048)):
     p0[:, 1] = np.rint(truth[:, 1])
     return dict(ref=sp.reference(2.0), tar=sp.target(field, 2.0), field=field, coords=coords, radius=20,
                 order=1, p0=p0, truth=truth)
+
+
+class ShearBand:
+    """A Gaussian shear band plus a vertical sinusoid.
+
+    The band's centre line runs through `centre` at `angle` degrees; with
+    t = (cos a, sin a) along it, n = (-sin a, cos a) across it and
+    d = (X - c)·n, the tangential slip is s(d) = slip·exp(-d²/(2σ²)) with
+    σ = width / (2√(2 ln 2)) (full width at half maximum = width).  Then
+        u = s(d)·t_x,    v = s(d)·t_y + amp·sin(2π X / period).
+    """
+
+    def __init__(self, centre, angle=30.0, width=60.0, slip=4.0, amp=0.8, period=512.0):
+        a = math.radians(angle)
+        self.c = np.asarray(centre, dtype=np.float64)
+        self.t = np.array([math.cos(a), math.sin(a)])
+        self.n = np.array([-math.sin(a), math.cos(a)])
+        self.sigma = width / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+        self.slip, self.amp, self.k = float(slip), float(amp), 2.0 * math.pi / float(period)
+        self.max_stretch = 1.0 + self.slip / self.sigma + self.amp * self.k
+
+    def _s(self, X, Y):
+        d = (X - self.c[0]) * self.n[0] + (Y - self.c[1]) * self.n[1]
+        s = self.slip * np.exp(-0.5 * (d / self.sigma) ** 2)
+        return d, s
+
+    def displacement(self, X, Y):
+        _, s = self._s(X, Y)
+        return s * self.t[0], s * self.t[1] + self.amp * np.sin(self.k * X)
+
+    def forward(self, cen):
+        u, v = self.displacement(cen[:, 0], cen[:, 1])
+        return cen + np.column_stack([u, v])
+
+    def inverse(self, xs, ys):
+        """X with X + u(X) = x, by fixed-point iteration to 1e-10 px."""
+        X, Y = np.array(xs, dtype=np.float64), np.array(ys, dtype=np.float64)
+        for _ in range(200):
+            u, v = self.displacement(X, Y)
+            Xn, Yn = xs - u, ys - v
+            delta = max(float(np.max(np.abs(Xn - X), initial=0.0)), float(np.max(np.abs(Yn - Y), initial=0.0)))
+            X, Y = Xn, Yn
+            if delta < 1e-10:
+                break
+        return X, Y
+
+    def warp(self, X, Y, order=2):
+        """True warp vector [u, v, u_x, v_x, u_y, v_y, u_xx, v_xx, u_xy, v_xy, u_yy, v_yy]."""
+        d, s = self._s(X, Y)
+        s1 = -s * d / self.sigma ** 2                          # ds/dd
+        s2 = s * (d * d / self.sigma ** 4 - 1.0 / self.sigma ** 2)  # d²s/dd²
+        nx, ny = self.n
+        tx, ty = self.t
+        kx = self.k * X
+        p = np.zeros((len(X), 12))
+        p[:, 0] = s * tx
+        p[:, 1] = s * ty + self.amp * np.sin(kx)
+        p[:, 2] = s1 * nx * tx
+        p[:, 3] = s1 * nx * ty + self.amp * self.k * np.cos(kx)
+        p[:, 4] = s1 * ny * tx
+        p[:, 5] = s1 * ny * ty
+        if order >= 2:
+            p[:, 6] = s2 * nx * nx * tx
+            p[:, 7] = s2 * nx * nx * ty - self.amp * self.k ** 2 * np.sin(kx)
+            p[:, 8] = s2 * nx * ny * tx
+            p[:, 9] = s2 * nx * ny * ty
+            p[:, 10] = s2 * ny * ny * tx
+            p[:, 11] = s2 * ny * ny * ty
+        return p
+
+
+def band_nodes(shape, field, n_target, margin, near=100.0):
+    """Adaptive node layout: a square grid over the image plus a second grid
+    offset by half a pitch within `near` px of the band's centre line (twice
+    the density there), pitch chosen so that about n_target nodes result."""
+    h, w = shape
+    area = (w - 2 * margin) * (h - 2 * margin)
+
+    def layout(pitch):
+        xs = np.arange(margin, w - margin + 1e-9, pitch)
+        ys = np.arange(margin, h - margin + 1e-9, pitch)
+        X, Y = np.meshgrid(xs, ys)
+        base = np.column_stack([X.ravel(), Y.ravel()])
+        off = base + 0.5 * pitch
+        off = off[(off[:, 0] <= w - margin) & (off[:, 1] <= h - margin)]
+        d = (off[:, 0] - field.c[0]) * field.n[0] + (off[:, 1] - field.c[1]) * field.n[1]
+        return np.vstack([base, off[np.abs(d) <= near]])
+
+    pitch = math.sqrt(area / n_target)
+    for _ in range(30):
+        pts = layout(pitch)
+        if abs(len(pts) - n_target) <= 0.002 * n_target:
+            break
+        pitch *= math.sqrt(len(pts) / n_target)
+    return np.rint(pts * 4.0) / 4.0  # quarter-pixel centres
+
+
+def shear_band_pair(seed=2, shape=(3000, 4096), n_speckles=150000, n_subsets=150000, radii=(15, 40)):
+    """Uneven subsets, second-order warp: a 4096 x 3000 pair with 150 000
+    speckles (σ ~ U[1.5, 3] px per speckle, noise σ = 2 grey levels) under a
+    Gaussian shear band (full width 60 px, 30°, peak tangential slip 4 px)
+    plus v += 0.8 sin(2πx/512); about 150 000 subsets on adaptive nodes
+    (twice the density within 100 px of the band) with radii uniform in
+    15..40 px; p0 = the true displacement rounded to whole pixels, zero
+    gradients.  Smaller shapes scale the speckle and subset counts."""
+    h, w = shape
+    scale = (h * w) / (3000.0 * 4096.0)
+    sp = Speckle(shape, max(1, int(round(n_speckles * scale))), (1.5, 3.0), seed)
+    field = ShearBand(((w - 1) / 2.0, (h - 1) / 2.0))
+    nsub = max(1, int(round(n_subsets * scale)))
+    rmax = radii[1]
+    coords = band_nodes(shape, field, nsub, margin=rmax + 6)
+    rr = _streams(seed, 7)[6]
+    r = rr.integers(radii[0], radii[1] + 1, len(coords))
+    truth = field.warp(coords[:, 0], coords[:, 1], order=2)
+    p0 = np.zeros((len(coords), 12))
+    p0[:, 0] = np.rint(truth[:, 0])
+    p0[:, 1] = np.rint(truth[:, 1])
+    return dict(ref=sp.reference(2.0), tar=sp.target(field, 2.0), field=field, coords=coords, radii=r,
+                order=2, p0=p0, truth=truth)

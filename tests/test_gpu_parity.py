@@ -170,3 +170,71 @@ def test_batch_deterministic_alone_and_in_mix(mesh_small):
         alone = gp.solve_subsets(f, g, w["coords"][k:k + 1], m, p0=w["p0"][k:k + 1], precision="f32")
         assert np.array_equal(alone["p"][0], full["p"][k])
         assert alone["iterations"][0] == full["iterations"][k]
+
+
+# ---- uneven radii, second-order warp (the batched-uneven workload at a
+# reduced size: 700 x 500 shear-band pair, radii 15..40, 300 subsets) ----
+
+@pytest.fixture(scope="module")
+def band_small():
+    w = workloads.shear_band_pair(seed=2, shape=(500, 700))
+    pick = np.random.Generator(np.random.PCG64(11)).choice(len(w["coords"]), 300, replace=False)
+    coords, radii, p0 = w["coords"][pick], w["radii"][pick], w["p0"][pick]
+    ref = w["ref"].astype(np.float64)
+    tar = w["tar"].astype(np.float64)
+    masks = {r: gp.Mask(mask_type="local", shape="circle", size=int(r)) for r in np.unique(radii)}
+    local = [masks[r] for r in radii]
+    index, start, xy = gp.templates_from_masks([m.coords for m in local])
+    n = len(coords)
+    orc = O.batch_solve(O.get_c(ref, 20), O.get_c(tar, 20), ref.shape, 20, coords, index, xy.astype(np.float64),
+                        start, np.full(n, 2, np.int32), p0, max_norm=1e-5, max_iterations=50)
+    return dict(coords=coords, radii=radii, p0=p0, ref=ref, tar=tar, local=local, orc=orc)
+
+
+def test_uneven_order2_plain_f64_matches_oracle(band_small):
+    b, orc = band_small, band_small["orc"]
+    f = gp.Image(image_gs=b["ref"], border=20)
+    g = gp.Image(image_gs=b["tar"], border=20)
+    r = gp.solve_subsets(f, g, b["coords"], b["local"], p0=b["p0"], subset_order=2, precision="f64",
+                         max_norm=1e-5)
+    assert np.array_equal(r["status"], orc["status"])
+    ok = orc["status"] == 0
+    du = np.abs(r["p"][ok, :2] - orc["p"][ok, :2]).max()
+    ds = np.abs(r["p"][ok, 2:] - orc["p"][ok, 2:]).max()
+    print("uneven order 2, plain f64: max |du|", du, "max |dp 2..11|", ds)
+    assert du < 1e-4 and ds < 1e-6
+    assert np.array_equal(r["converged"], orc["converged"])
+    assert np.array_equal(r["iterations"], orc["iterations"])
+
+
+def test_uneven_order2_f32_engine_matches_oracle(band_small):
+    b, orc = band_small, band_small["orc"]
+    f = gp.Image(image_gs=b["ref"], border=20)
+    g = gp.Image(image_gs=b["tar"], border=20)
+    r = gp.solve_subsets(f, g, b["coords"], b["local"], p0=b["p0"], subset_order=2, precision="f32",
+                         max_norm=1e-5)
+    both = (r["converged"] == 1) & (orc["converged"] == 1)
+    du = np.abs(r["p"][both, :2] - orc["p"][both, :2]).max()
+    ds = np.abs(r["p"][both, 2:6] - orc["p"][both, 2:6]).max()
+    agree = np.mean(r["converged"] == orc["converged"])
+    print("uneven order 2, f32 engine: max |du|", du, "max |dstrain|", ds, "convergence agreement", agree,
+          "converged", orc["converged"].mean())
+    assert agree >= 0.98
+    assert du < 1e-4
+    assert ds < 1e-5
+    np.testing.assert_allclose(r["c_zncc"][both], orc["c_zncc"][both], atol=1e-5)
+    np.testing.assert_allclose(r["eta_u"][both], orc["eta_u"][both], rtol=2e-2, atol=1e-6)
+
+
+def test_uneven_size_classes_alone_and_in_mix(band_small):
+    b = band_small
+    f = gp.Image(image_gs=b["ref"], border=20)
+    g = gp.Image(image_gs=b["tar"], border=20)
+    full = gp.solve_subsets(f, g, b["coords"], b["local"], p0=b["p0"], subset_order=2, precision="f32")
+    # one subset from each size class (1, 2, 4 and 8 wavefronts)
+    for r_want in (15, 20, 28, 40):
+        k = int(np.argmin(np.abs(b["radii"] - r_want)))
+        alone = gp.solve_subsets(f, g, b["coords"][k:k + 1], [b["local"][k]], p0=b["p0"][k:k + 1],
+                                 subset_order=2, precision="f32")
+        assert np.array_equal(alone["p"][0], full["p"][k])
+        assert alone["iterations"][0] == full["iterations"][k]
