@@ -49,6 +49,87 @@ def measured_traffic():
     return None
 
 
+def run_shear_band(args, torch, gp, _native, workloads):
+    """Batched uneven subsets, second-order warp: 4096 x 3000 shear-band
+    pair, ~150 000 subsets with radii 15..40 on adaptive nodes, all in one
+    frame of concurrent size-class launches under the autotuned schedule."""
+    t0 = time.time()
+    w = workloads.shear_band_pair(seed=2)
+    log(f"workload generated in {time.time() - t0:.1f}s")
+    ref, tar, coords, radii, p0, truth = w["ref"], w["tar"], w["coords"], w["radii"], w["p0"], w["truth"]
+    n = len(coords)
+    border = 20
+    eng = gp._engine()
+    f_img = _native.DeviceImage(eng, pixels=ref, border=border)
+    dev_tar = torch.from_numpy(tar).cuda()
+    torch.cuda.synchronize()
+    g_img = _native.DeviceImage(eng, device_ptr=dev_tar.data_ptr(), shape=tar.shape, pix_type=_native.PIX_U8,
+                                border=border)
+    masks = {int(r): gp.Mask(mask_type="local", shape="circle", size=int(r)) for r in np.unique(radii)}
+    index, start, xy = gp.templates_from_masks([masks[int(r)].coords for r in radii])
+    batch = _native.Batch(eng, coords, 2, index, start, np.rint(xy).astype(np.int16))
+    batch.set_p0(p0)
+    if not args.no_autotune:
+        t = time.time()
+        batch.autotune(f_img, g_img, max_iterations=args.max_iterations, max_norm=1e-5)
+        log(f"autotuned in {time.time() - t:.1f}s")
+
+    def frame():
+        g_img.update(device_ptr=dev_tar.data_ptr(), pix_type=_native.PIX_U8)
+        batch.solve(f_img, g_img, method="icgn", precision=_native.PREC_COMP_F32,
+                    max_iterations=args.max_iterations, tolerance=0.75, max_norm=1e-5)
+        eng.sync()
+
+    for _ in range(args.warmup):
+        frame()
+    lat, sol = [], []
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        frame()
+        lat.append((time.perf_counter() - t) * 1e3)
+        sol.append(eng.timing()[1])
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - tb
+    res = batch.results()
+    out = {
+        "metric": "ICGN subsets/sec, uneven radii 15-40 px, second-order warp (FP32 engine)",
+        "value": round(n / (total_s / args.steps), 1), "unit": "subsets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total_s * 1e3 / args.steps, 4), "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "shear band: 4096x3000 pair, 150000 speckles sigma U[1.5,3], band width 60 px at "
+                               "30 deg, slip 4 px, v += 0.8 sin(2 pi x/512); adaptive nodes, radii U{15..40}, "
+                               "order 2", "subsets": n, "image": list(ref.shape), "order": 2},
+        "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                             "p99": round(float(np.percentile(lat, 99)), 4)},
+        "kernel_ms": round(float(np.mean(sol)), 4),
+        "iterations_mean": round(float(np.mean(res["iterations"])), 3),
+        "converged_fraction": round(float(np.mean(res["converged"])), 5),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        idx = np.linspace(0, n - 1, min(1000, n)).astype(np.int64)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        Cf = O.get_c(ref.astype(np.float64), border, threads)
+        Cg = O.get_c(tar.astype(np.float64), border, threads)
+        t = O.wall()
+        orc = O.batch_solve(Cf, Cg, ref.shape, border, coords[idx], index[idx], xy.astype(np.float64), start,
+                            np.full(len(idx), 2, np.int32), p0[idx], max_norm=1e-5, max_iterations=50,
+                            threads=threads)
+        cpu_s = O.wall() - t
+        d = res[idx]
+        both = (d["converged"] == 1) & (orc["converged"] == 1)
+        out["cpu_baseline"] = {"value": round(len(idx) / cpu_s, 2), "unit": "subsets/s", "cores": threads,
+                               "kind": "port", "sample": f"{len(idx)} evenly strided subsets, {cpu_s:.1f} s"}
+        out["error_vs_oracle"] = {"max_du_px": float(np.abs(d["p"][both, :2] - orc["p"][both, :2]).max()),
+                                  "max_dstrain": float(np.abs(d["p"][both, 2:6] - orc["p"][both, 2:6]).max()),
+                                  "converged_match": float(np.mean(d["converged"] == orc["converged"])),
+                                  "sample": len(idx)}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +148,9 @@ def main():
                     help="only run a 512 MiB device-to-device copy (known HBM bytes) for counter scaling")
     ap.add_argument("--engine", choices=["both", "f32"], default="both",
                     help="f32: time only the batched engine (for profiler runs)")
+    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band"], default="uniform_mesh",
+                    help="shear_band: the uneven-radius, second-order workload (its own line, not the headline)")
+    ap.add_argument("--no-autotune", action="store_true", help="shear_band: keep the default schedule")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU "
@@ -85,6 +169,10 @@ def main():
         return
     import geopyv_dev as gp
     from geopyv_dev import _native, workloads
+
+    if args.workload == "shear_band":
+        run_shear_band(args, torch, gp, _native, workloads)
+        return
 
     t0 = time.time()
     w = workloads.uniform_mesh_pair(seed=1)

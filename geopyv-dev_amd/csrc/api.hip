@@ -555,6 +555,29 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
     return DIC_OK;
 }
 
+int dic_batch_autotune(dic_batch* db, const dic_image* f, const dic_image* g, const dic_solve_params* prm) {
+    CHECK_ARG(db && f && g && prm, "dic_batch_autotune: NULL argument");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    CHECK_ARG(f->im.eng == e && g->im.eng == e, "invalid input: images belong to another engine");
+    CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
+              "invalid input: reference and target images differ in shape or border");
+    CHECK_ARG(prm->max_iterations >= 0 && prm->max_iterations <= 100000, "invalid input: max_iterations");
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    int st = refresh_order(b, f->im.w, f->im.h);
+    if (st) return st;
+    dic_solve_params p = *prm;
+    p.want_history = 0;
+    p.precision = DIC_PREC_COMP_F32;
+    p.method = DIC_METHOD_ICGN;
+    st = autotune_icgn_batch(b, &f->im, &g->im, &p);
+    if (st) return st;
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+
 int dic_batch_results(dic_batch* db, dic_subset_result* out) {
     CHECK_ARG(db && out, "dic_batch_results: NULL argument");
     Batch* b = &db->b;

@@ -64,7 +64,13 @@ std::vector<SizeClass> size_classes(const Batch* b) {
         const int w = b->waves_per_subset > 0 ? b->waves_per_subset : min_wps(npx(b->h_perm[i]));
         SizeClass c;
         c.start = i;
+        c.base_wps = w;
         c.wps = w;
+        c.slack = b->win_slack;
+        if (b->waves_per_subset == 0 && b->tuned_wps[w] > 0) {
+            c.wps = b->tuned_wps[w];
+            c.slack = b->tuned_slack[w];
+        }
         c.max_n = 0;
         c.max_side = 0;
         while (i < b->n && (b->waves_per_subset > 0 || min_wps(npx(b->h_perm[i])) == w)) {
@@ -78,8 +84,9 @@ std::vector<SizeClass> size_classes(const Batch* b) {
     return out;
 }
 
-int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
-    Engine* e = b->eng;
+namespace {
+
+BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
     BatchArgs A;
     A.fC = f->coef32;
     A.gC = g->coef32;
@@ -102,13 +109,36 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.max_it = prm->max_iterations;
     A.tol = prm->tolerance;
     A.max_norm = prm->max_norm;
-    A.win_slack = b->win_slack;
+    return A;
+}
+
+int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t st) {
+    if (c.max_n > 64 * 12 * c.wps) {
+        set_error("invalid input: subset larger than the wavefronts per subset can hold");
+        return DIC_ERR_INVALID_INPUT;
+    }
+    BatchArgs Ac = A;
+    Ac.perm = b->d_order_perm + c.start;
+    Ac.n_sub = c.count;
+    Ac.win_slack = c.slack;
+    // LDS: the coefficient window (template box, slack, taps, cell rounding
+    // and a little strain), then the template offsets as FP16 pairs
+    const int side = c.max_side + 2 * c.slack + 8;
+    const int row = (side + 3 + 3) & ~3;  // plus the 16-byte rounding of the first column
+    const bool s72 = row <= 72;
+    Ac.win_cap = (s72 ? 72 : row) * side;
+    const size_t lds = sizeof(float) * (size_t)Ac.win_cap + 4 * (size_t)c.max_n;
+    if (b->max_order == 0) return launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st);
+    if (b->max_order == 1) return launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st);
+    return launch_m<12>(Ac, c.wps, (int)c.count, lds, s72, st);
+}
+
+}  // namespace
+
+int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+    Engine* e = b->eng;
+    const BatchArgs A = common_args(b, f, g, prm);
     const std::vector<SizeClass> cls = size_classes(b);
-    for (const SizeClass& c : cls)
-        if (c.max_n > 64 * 12 * c.wps) {
-            set_error("invalid input: subset larger than the wavefronts per subset can hold");
-            return DIC_ERR_INVALID_INPUT;
-        }
     // the first class runs on the engine's stream, the others on the
     // auxiliary streams, forked from and joined back into it
     const int nstreams = std::min<int>((int)cls.size(), 1 + Engine::kAux);
@@ -117,26 +147,8 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
         for (int k = 0; k + 1 < nstreams; ++k) DIC_HIP_TRY(hipStreamWaitEvent(e->aux[k], e->ev_fork, 0));
     }
     for (size_t k = 0; k < cls.size(); ++k) {
-        const SizeClass& c = cls[k];
-        BatchArgs Ac = A;
-        Ac.perm = b->d_order_perm + c.start;
-        Ac.n_sub = c.count;
-        // LDS: the coefficient window (template box, slack, taps, cell
-        // rounding and a little strain), then the template offsets as FP16
-        // pairs
-        const int side = c.max_side + 2 * b->win_slack + 8;
-        const int row = (side + 3 + 3) & ~3;  // plus the 16-byte rounding of the first column
-        const bool s72 = row <= 72;
-        Ac.win_cap = (s72 ? 72 : row) * side;
-        const size_t lds = sizeof(float) * (size_t)Ac.win_cap + 4 * (size_t)c.max_n;
         hipStream_t st = (k % nstreams == 0) ? e->stream : e->aux[k % nstreams - 1];
-        int r;
-        if (b->max_order == 0)
-            r = launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st);
-        else if (b->max_order == 1)
-            r = launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st);
-        else
-            r = launch_m<12>(Ac, c.wps, (int)c.count, lds, s72, st);
+        const int r = launch_class(b, A, cls[k], st);
         if (r) return r;
     }
     for (int k = 0; k + 1 < nstreams; ++k) {
@@ -144,6 +156,56 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
         DIC_HIP_TRY(hipStreamWaitEvent(e->stream, e->ev_join[k], 0));
     }
     return DIC_OK;
+}
+
+// Every class alone on the engine's stream, for each candidate (fewest
+// wavefronts or twice that, window slack 1 or 3 pixels), best of two
+// timed runs; the fastest is kept for later solves.  The results the
+// batch holds afterwards are those of the last candidate.
+int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+    Engine* e = b->eng;
+    const BatchArgs A = common_args(b, f, g, prm);
+    for (int w = 0; w < 9; ++w) b->tuned_wps[w] = 0;
+    const int saved = b->waves_per_subset;
+    b->waves_per_subset = 0;
+    const std::vector<SizeClass> cls = size_classes(b);
+    b->waves_per_subset = saved;
+    hipEvent_t t0, t1;
+    DIC_HIP_TRY(hipEventCreate(&t0));
+    DIC_HIP_TRY(hipEventCreate(&t1));
+    int rc = DIC_OK;
+    for (const SizeClass& c0 : cls) {
+        float best = 1e30f;
+        int bw = c0.base_wps, bs = b->win_slack;
+        for (int w = c0.base_wps; w <= std::min(8, 2 * c0.base_wps) && !rc; w *= 2)
+            for (int sl : {1, 3}) {
+                SizeClass c = c0;
+                c.wps = w;
+                c.slack = sl;
+                float tmin = 1e30f;
+                for (int rep = 0; rep < 2 && !rc; ++rep) {
+                    if (hipEventRecord(t0, e->stream) != hipSuccess) rc = DIC_ERR_DEVICE;
+                    if (!rc) rc = launch_class(b, A, c, e->stream);
+                    if (!rc && hipEventRecord(t1, e->stream) != hipSuccess) rc = DIC_ERR_DEVICE;
+                    float ms = 0.f;
+                    if (!rc && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+                        rc = DIC_ERR_DEVICE;
+                    tmin = std::min(tmin, ms);
+                }
+                if (!rc && tmin < best) {
+                    best = tmin;
+                    bw = w;
+                    bs = sl;
+                }
+            }
+        if (rc) break;
+        b->tuned_wps[c0.base_wps] = bw;
+        b->tuned_slack[c0.base_wps] = bs;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc == DIC_ERR_DEVICE) set_error("autotune: device error while timing a candidate");
+    return rc;
 }
 
 }  // namespace geodic

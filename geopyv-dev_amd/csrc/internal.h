@@ -74,6 +74,8 @@ struct Batch {
     int32_t max_side = 0;           // largest template box side
     int32_t win_slack = 3;          // deformation slack of the LDS window (pixels per side)
     int32_t waves_per_subset = 0;   // batched engine: wavefronts per subset (0 = smallest that fits)
+    int32_t tuned_wps[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};    // autotuned wavefronts by a class's fewest (0 = none)
+    int32_t tuned_slack[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int64_t* d_scr_off = nullptr;   // n: offset of each subset's scratch
     double* d_p0 = nullptr;         // n x 12
     double* d_p = nullptr;          // n x 12 final warps
@@ -104,12 +106,16 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
 // icgn_batch.hip
 struct SizeClass {
     int64_t start, count;  // run of the dispatch order
-    int wps;               // wavefronts per subset
+    int base_wps;          // fewest wavefronts that hold the class's subsets
+    int wps;               // wavefronts per subset used
+    int slack;             // LDS window slack used
     int64_t max_n;
     int max_side;
 };
 std::vector<SizeClass> size_classes(const Batch* b);
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
+// times the candidate schedules of every size class and keeps the fastest
+int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 
 // device memory accounting
 int dev_alloc(Engine* e, void** p, size_t bytes);

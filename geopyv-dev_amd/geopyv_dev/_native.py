@@ -38,7 +38,7 @@ EXPORTED = [
     "dic_engine_bytes_in_use", "dic_image_create", "dic_image_update", "dic_image_coef", "dic_image_info",
     "dic_image_destroy", "dic_mask_local", "dic_mask_update", "dic_batch_create", "dic_batch_set_p0",
     "dic_batch_solve", "dic_batch_results", "dic_batch_history", "dic_batch_subset_arrays",
-    "dic_batch_set_schedule", "dic_batch_device_p", "dic_batch_destroy",
+    "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
 ]
 
 
@@ -114,6 +114,7 @@ def load():
         L.dic_batch_history.argtypes = [_vp, _vp]
         L.dic_batch_subset_arrays.argtypes = [_vp, ctypes.c_int64, _vp, _vp]
         L.dic_batch_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.dic_batch_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
         L.dic_batch_device_p.argtypes = [_vp, ctypes.POINTER(_vp)]
         L.dic_batch_destroy.argtypes = [_vp]
         L.dic_batch_destroy.restype = None
@@ -280,6 +281,12 @@ class Batch:
 
     def set_schedule(self, win_slack, waves_per_subset):
         check(load().dic_batch_set_schedule(self.handle, int(win_slack), int(waves_per_subset)))
+
+    def autotune(self, f_img, g_img, max_iterations=50, tolerance=0.75, max_norm=1e-5):
+        """Time the batched engine's candidate schedules per size class and
+        keep the fastest (results are unspecified until the next solve)."""
+        prm = SolveParams(METHOD["icgn"], PREC_COMP_F32, int(max_iterations), 0, float(tolerance), float(max_norm))
+        check(load().dic_batch_autotune(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm)))
 
     def solve(self, f_img, g_img, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50, tolerance=0.75,
               max_norm=1e-5, history=False):

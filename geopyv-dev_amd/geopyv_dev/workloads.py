@@ -81,7 +81,17 @@ class Speckle:
         return _quantise(self.clean_ref, self.scale, noise_sigma, self.rn_ref)
 
     def target(self, field, noise_sigma, rng=None):
-        clean = _render(self.shape, self.centres, self.amps, self.sigmas, inv_map=field.inverse,
+        inv = field.inverse
+        if hasattr(field, "inverse_grid"):
+            # the map is evaluated at integer pixels only: invert once per pixel
+            h, w = self.shape
+            Xg, Yg = field.inverse_grid(self.shape)
+
+            def inv(xs, ys):
+                ix = np.clip(xs.astype(np.int64), 0, w - 1)
+                iy = np.clip(ys.astype(np.int64), 0, h - 1)
+                return Xg[iy, ix], Yg[iy, ix]
+        clean = _render(self.shape, self.centres, self.amps, self.sigmas, inv_map=inv,
                         fwd_centre=field.forward, max_stretch=field.max_stretch)
         return _quantise(clean, self.scale, noise_sigma, rng if rng is not None else self.rn_tar)
 
@@ -223,6 +233,20 @@ class ShearBand:
             X, Y = Xn, Yn
             if delta < 1e-10:
                 break
+        return X, Y
+
+    def inverse_grid(self, shape):
+        """The inverse map at every pixel of an (h, w) grid, row by row."""
+        h, w = shape
+        X = np.empty((h, w))
+        Y = np.empty((h, w))
+        xs = np.arange(w, dtype=np.float64)
+        for i in range(0, h, 256):
+            ys = np.arange(i, min(h, i + 256), dtype=np.float64)
+            gx, gy = np.meshgrid(xs, ys)
+            a, b = self.inverse(gx, gy)
+            X[i:i + len(ys)] = a
+            Y[i:i + len(ys)] = b
         return X, Y
 
     def warp(self, X, Y, order=2):

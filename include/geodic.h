@@ -140,11 +140,19 @@ int dic_batch_history(dic_batch* b, double* out);
  * subset (subset.rs:877-884); available after a DIC_PREC_PLAIN_F64 solve. */
 int dic_batch_subset_arrays(dic_batch* b, int64_t subset, double* f, double* grad_f);
 /* Schedule of the batched engine (DIC_PREC_COMP_F32): LDS window slack in
- * pixels per side beyond the template box, and wavefronts per subset (1, 2,
- * 4 or 8; 0 picks the fewest that hold the largest subset at 12 pixels per
- * thread).  The autotuner in the host package picks these; no reference
- * counterpart (the reference has no device schedule). */
+ * pixels per side beyond the template box, and wavefronts per subset for the
+ * whole batch (1, 2, 4 or 8, overriding the autotuned choice; 0 = per size
+ * class: the autotuned choice, else the fewest wavefronts that hold the
+ * class's subsets at 12 pixels per thread).  No reference counterpart (the
+ * reference has no device schedule). */
 int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_subset);
+/* Autotuner of the batched engine: for each size class of the batch (subsets
+ * needing 1, 2, 4 or 8 wavefronts) times the candidate schedules (that many
+ * wavefronts or twice as many; window slack 1 or 3 px) on these images and
+ * keeps the fastest for later DIC_PREC_COMP_F32 solves.  Results held by the
+ * batch afterwards are unspecified until the next solve.  No reference
+ * counterpart (the reference has no device schedule). */
+int dic_batch_autotune(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
 /* Final warps as a device pointer (n_subsets x 12 doubles) for chaining
  * batches without a host round trip. */
 int dic_batch_device_p(dic_batch* b, const double** out);
