@@ -47,6 +47,13 @@ struct dic_image {
 struct dic_batch {
     Batch b;
 };
+struct dic_frame_graph {
+    Engine* e = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double* tmp = nullptr;  // the graph's own prefilter row-pass buffer
+    size_t tmp_bytes = 0;
+};
 
 #define CHECK_ARG(cond, msg)                  \
     do {                                      \
@@ -237,6 +244,7 @@ void dic_image_destroy(dic_image* di) {
     const size_t np = (size_t)im->hp * im->wp;
     dev_free(e, im->coef64, np * sizeof(double));
     dev_free(e, im->coef32, (size_t)im->hp * im->wp32 * sizeof(float));
+    dev_free(e, im->pix_dev, im->pix_dev_bytes);
     delete di;
 }
 
@@ -437,6 +445,19 @@ int dic_batch_set_p0(dic_batch* db, const double* p0, int mem) {
     return DIC_OK;
 }
 
+int dic_batch_reset_warps(dic_batch* db, const double* p, int mem) {
+    CHECK_ARG(db && p, "dic_batch_reset_warps: NULL argument");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    const hipMemcpyKind k = mem == DIC_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    DIC_HIP_TRY(hipMemcpyAsync(b->d_p0, p, sizeof(double) * 12 * b->n, k, e->stream));
+    DIC_HIP_TRY(hipMemcpyAsync(b->d_p, p, sizeof(double) * 12 * b->n, k, e->stream));
+    if (mem == DIC_MEM_HOST) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
 int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_subset) {
     CHECK_ARG(db, "dic_batch_set_schedule: NULL batch");
     CHECK_ARG(win_slack >= 0 && win_slack <= 64, "invalid input: win_slack must be in 0..64");
@@ -508,6 +529,24 @@ int refresh_order(Batch* b, int64_t w, int64_t h) {
 
 }  // namespace
 
+namespace {
+
+// The solve's device work on the engine's stream (timed by the engine's
+// events unless a graph is being recorded).
+int solve_enqueue(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+    Engine* e = b->eng;
+    if (!e->capturing) DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
+    int st = prm->precision == DIC_PREC_PLAIN_F64 ? launch_icgn_plain(b, f, g, prm) : launch_icgn_batch(b, f, g, prm);
+    if (st) return st;
+    if (!e->capturing) {
+        DIC_HIP_TRY(hipEventRecord(e->ev_sol1, e->stream));
+        e->sol_recorded = true;
+    }
+    return DIC_OK;
+}
+
+}  // namespace
+
 int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const dic_solve_params* prm) {
     CHECK_ARG(db && f && g && prm, "dic_batch_solve: NULL argument");
     Batch* b = &db->b;
@@ -537,17 +576,8 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
     }
     int st = refresh_order(b, f->im.w, f->im.h);
     if (st) return st;
-    if (prm->precision == DIC_PREC_PLAIN_F64) {
-        // scratch sizing and offsets are host work done before the timed launch
-        DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
-        st = launch_icgn_plain(b, &f->im, &g->im, prm);
-    } else {
-        DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
-        st = launch_icgn_batch(b, &f->im, &g->im, prm);
-    }
+    st = solve_enqueue(b, &f->im, &g->im, prm);
     if (st) return st;
-    DIC_HIP_TRY(hipEventRecord(e->ev_sol1, e->stream));
-    e->sol_recorded = true;
     b->last_precision = prm->precision;
     b->last_method = prm->method;
     b->last_max_it = prm->max_iterations;
@@ -641,6 +671,115 @@ void dic_batch_destroy(dic_batch* db) {
     (void)hipStreamSynchronize(b->eng->stream);
     batch_free(b);
     delete db;
+}
+
+int dic_image_stage(dic_image* di, const void* pixels, int pix_type, int mem) {
+    CHECK_ARG(di && pixels, "dic_image_stage: NULL argument");
+    CHECK_ARG(pix_type == DIC_PIX_U8 || pix_type == DIC_PIX_F64, "invalid input: pixel type");
+    CHECK_ARG(mem == DIC_MEM_HOST || mem == DIC_MEM_DEVICE, "invalid input: memory kind");
+    Image* im = &di->im;
+    Engine* e = im->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    const size_t bytes = (size_t)im->h * im->w * (pix_type == DIC_PIX_U8 ? 1 : 8);
+    // a frame graph may hold this buffer: its type (and so its size) is
+    // fixed by the first staging
+    CHECK_ARG(!im->pix_dev || im->pix_dev_type == pix_type,
+              "invalid input: staged pixel type differs from the first staging");
+    if (!im->pix_dev) {
+        int st = dev_alloc(e, &im->pix_dev, bytes);
+        if (st) return st;
+        im->pix_dev_bytes = bytes;
+        im->pix_dev_type = pix_type;
+    }
+    DIC_HIP_TRY(hipMemcpyAsync(im->pix_dev, pixels, bytes,
+                               mem == DIC_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, e->stream));
+    if (mem == DIC_MEM_HOST) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_frame_graph_create(dic_batch* db, const dic_image* f, dic_image* g, int warm_start,
+                           const dic_solve_params* prm, dic_frame_graph** out) {
+    CHECK_ARG(db && f && g && prm && out, "dic_frame_graph_create: NULL argument");
+    *out = nullptr;
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    CHECK_ARG(f->im.eng == e && g->im.eng == e, "invalid input: images belong to another engine");
+    CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
+              "invalid input: reference and target images differ in shape or border");
+    CHECK_ARG(g->im.pix_dev != nullptr, "invalid input: stage the target's pixels (dic_image_stage) first");
+    CHECK_ARG(prm->precision == DIC_PREC_COMP_F32 && prm->method == DIC_METHOD_ICGN,
+              "invalid input: frame graphs run the batched engine (DIC_PREC_COMP_F32, ICGN)");
+    CHECK_ARG(!prm->want_history, "invalid input: frame graphs record no history");
+    CHECK_ARG(prm->max_iterations >= 0 && prm->max_iterations <= 100000, "invalid input: max_iterations");
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    Image* gi = &g->im;
+    // everything that allocates or synchronises happens before the capture
+    int st = refresh_order(b, f->im.w, f->im.h);
+    if (st) return st;
+    dic_frame_graph* gr = new dic_frame_graph();
+    gr->e = e;
+    gr->tmp_bytes = (size_t)gi->hp * gi->wp * sizeof(double);
+    st = dev_alloc(e, (void**)&gr->tmp, gr->tmp_bytes);
+    if (!st) st = prefilter_run(e, gi, gi->pix_dev, gi->pix_dev_type, DIC_MEM_DEVICE, gr->tmp);  // twiddles, attributes
+    if (!st) st = solve_enqueue(b, &f->im, gi, prm);
+    if (!st && hipStreamSynchronize(e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
+    if (!st) {
+        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_error("dic_frame_graph_create: stream capture could not start");
+            st = DIC_ERR_DEVICE;
+        } else {
+            e->capturing = true;
+            st = prefilter_run(e, gi, gi->pix_dev, gi->pix_dev_type, DIC_MEM_DEVICE, gr->tmp);
+            if (!st && warm_start &&
+                hipMemcpyAsync(b->d_p0, b->d_p, sizeof(double) * 12 * b->n, hipMemcpyDeviceToDevice, e->stream) !=
+                    hipSuccess)
+                st = DIC_ERR_DEVICE;
+            if (!st) st = solve_enqueue(b, &f->im, gi, prm);
+            e->capturing = false;
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+            if (!st && (ce != hipSuccess || !graph)) {
+                set_error("dic_frame_graph_create: stream capture failed");
+                st = DIC_ERR_DEVICE;
+            }
+            gr->graph = graph;
+            if (!st && hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0) != hipSuccess) {
+                set_error("dic_frame_graph_create: graph instantiation failed");
+                st = DIC_ERR_DEVICE;
+            }
+        }
+    }
+    if (st) {
+        if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+        if (gr->graph) (void)hipGraphDestroy(gr->graph);
+        dev_free(e, gr->tmp, gr->tmp_bytes);
+        delete gr;
+        return st;
+    }
+    *out = gr;
+    return DIC_OK;
+}
+
+int dic_frame_graph_launch(dic_frame_graph* gr) {
+    CHECK_ARG(gr, "dic_frame_graph_launch: NULL graph");
+    Engine* e = gr->e;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    DIC_HIP_TRY(hipGraphLaunch(gr->exec, e->stream));
+    return DIC_OK;
+}
+
+void dic_frame_graph_destroy(dic_frame_graph* gr) {
+    if (!gr) return;
+    Engine* e = gr->e;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    dev_free(e, gr->tmp, gr->tmp_bytes);
+    delete gr;
 }
 
 }  // extern "C"

@@ -35,6 +35,7 @@ struct Engine {
     bool pre_recorded = false, sol_recorded = false;
     int64_t bytes_in_use = 0;
     int num_cus = 256;
+    bool capturing = false;  // recording a graph: no timing events, no host synchronisation
     std::mutex mu;
     // cached twiddle tables per FFT length (device, complex double)
     std::vector<std::pair<int, double2*>> twiddles;
@@ -54,6 +55,9 @@ struct Image {
     int64_t wp32 = 0;        // row pitch of the FP32 copy: wp rounded up to 4 floats (16-byte rows)
     double* coef64 = nullptr;
     float* coef32 = nullptr;
+    void* pix_dev = nullptr;  // staged frame pixels (dic_image_stage), h x w of pix_type
+    size_t pix_dev_bytes = 0;
+    int pix_dev_type = -1;
 };
 
 struct Batch {
@@ -100,7 +104,9 @@ struct Batch {
 };
 
 // prefilter.hip
-int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem);
+// tmp_own: a caller-owned row-pass buffer (hp x wp doubles) instead of the
+// engine's, for work captured into graphs
+int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own = nullptr);
 // icgn_plain.hip
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 // icgn_batch.hip

@@ -212,7 +212,7 @@ int get_twiddles(Engine* e, int n, const double2** out) {
 
 }  // namespace
 
-int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem) {
+int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own) {
     const int64_t h = img->h, w = img->w, hp = img->hp, wp = img->wp;
     if (wp > (int64_t)kMaxOut * 1024 || hp > (int64_t)kMaxOut * 1024) {
         set_error("prefilter: padded line longer than 8192 samples");
@@ -232,7 +232,7 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem)
         dpix = e->pix_stage;
     }
     const size_t tmp_bytes = (size_t)hp * wp * sizeof(double);
-    if (e->pre_tmp_bytes < tmp_bytes) {
+    if (!tmp_own && e->pre_tmp_bytes < tmp_bytes) {
         if (e->pre_tmp) DIC_HIP_TRY(hipFree(e->pre_tmp));
         e->pre_tmp = nullptr;
         e->pre_tmp_bytes = 0;
@@ -251,29 +251,32 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem)
     }
     const int tr = threads_for((int)wp), tc = threads_for((int)hp);
     const size_t lr = sizeof(double2) * wp, lc = sizeof(double2) * hp;
-    DIC_HIP_TRY(hipEventRecord(e->ev_pre0, e->stream));
+    double* tmp = tmp_own ? tmp_own : e->pre_tmp;
+    if (!e->capturing) DIC_HIP_TRY(hipEventRecord(e->ev_pre0, e->stream));
     const int row_blocks = (int)((hp + 1) / 2);
     if (pix_type == DIC_PIX_U8) {
         DIC_HIP_TRY(hipFuncSetAttribute((const void*)prefilter_rows_kernel<uint8_t>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
         prefilter_rows_kernel<uint8_t><<<row_blocks, tr, lr, e->stream>>>((const uint8_t*)dpix, h, w, img->border, hp,
-                                                                          wp, sr, tw_r, e->pre_tmp);
+                                                                          wp, sr, tw_r, tmp);
     } else {
         DIC_HIP_TRY(hipFuncSetAttribute((const void*)prefilter_rows_kernel<double>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr));
         prefilter_rows_kernel<double><<<row_blocks, tr, lr, e->stream>>>((const double*)dpix, h, w, img->border, hp,
-                                                                         wp, sr, tw_r, e->pre_tmp);
+                                                                         wp, sr, tw_r, tmp);
     }
     DIC_HIP_TRY(hipGetLastError());
     const int npairs = (int)((wp + 1) / 2);
     const int per_xcd = (npairs + 7) / 8;
     DIC_HIP_TRY(hipFuncSetAttribute((const void*)prefilter_cols_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lc));
-    prefilter_cols_kernel<<<per_xcd * 8, tc, lc, e->stream>>>(e->pre_tmp, hp, wp, npairs, per_xcd, sc, tw_c,
+    prefilter_cols_kernel<<<per_xcd * 8, tc, lc, e->stream>>>(tmp, hp, wp, npairs, per_xcd, sc, tw_c,
                                                               img->coef64, img->coef32, img->wp32);
     DIC_HIP_TRY(hipGetLastError());
-    DIC_HIP_TRY(hipEventRecord(e->ev_pre1, e->stream));
-    e->pre_recorded = true;
+    if (!e->capturing) {
+        DIC_HIP_TRY(hipEventRecord(e->ev_pre1, e->stream));
+        e->pre_recorded = true;
+    }
     return DIC_OK;
 }
 

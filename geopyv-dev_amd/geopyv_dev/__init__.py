@@ -525,3 +525,58 @@ def solve_subsets(f_img, g_img, coords, local_mask, p0=None, subset_order=1, pre
     b.solve(f_img._dev, g_img._dev, method=method, precision=prec, max_iterations=max_iterations,
             tolerance=tolerance, max_norm=max_norm)
     return b.results()
+
+
+def solve_sequence(frames, coords, local_mask, subset_order=1, reference_every=20, p0=None, border=20,
+                   max_norm=1e-5, max_iterations=50, tolerance=0.75, use_graph=True):
+    """Incremental correlation over a sequence of grey frames, the frame loop
+    of sequence.rs:624-968 in batch form: frames[0] is the first reference;
+    each later frame is prefiltered on the device and its subsets solved,
+    warm-started from the previous frame's warps; every `reference_every`
+    frames the current frame becomes the reference (its coefficients are
+    recomputed on the device, sequence.rs:914) and the warps restart from
+    zero.  The per-frame steps run as one captured HIP graph (use_graph) or
+    as eager calls; both give identical results.
+
+    Returns one result array per frame after the first (fields as
+    dic_subset_result, warps relative to that frame's reference)."""
+    frames = list(frames)
+    if len(frames) < 2:
+        raise ValueError("a sequence needs at least two frames")
+    coords = np.asarray(coords, dtype=np.float64).reshape(-1, 2)
+    n = len(coords)
+    if isinstance(local_mask, (list, tuple)):
+        index, start, xy = templates_from_masks([m.coords for m in local_mask])
+    else:
+        xy = np.rint(local_mask.coords).astype(np.int16)
+        index = np.zeros(n, dtype=np.int32)
+        start = np.array([0, len(xy)], dtype=np.int64)
+    eng = _engine()
+    first = np.asarray(frames[0])
+    f = _native.DeviceImage(eng, pixels=first, border=border)
+    g = _native.DeviceImage(eng, pixels=np.asarray(frames[1]), border=border)
+    b = _native.Batch(eng, coords, subset_order, index, start, xy)
+    b.reset_warps(np.zeros((n, 12)) if p0 is None else np.asarray(p0, dtype=np.float64).reshape(n, 12))
+    out = []
+    graph = None
+    for k in range(1, len(frames)):
+        frame = np.asarray(frames[k])
+        if use_graph:
+            g.stage(frame)
+            if graph is None:
+                graph = _native.FrameGraph(b, f, g, warm_start=True, max_iterations=max_iterations,
+                                           tolerance=tolerance, max_norm=max_norm)  # runs this frame
+            else:
+                graph.launch()
+        else:
+            if k > 1:
+                b.set_p0_device(b.device_p())
+            g.update(frame)
+            b.solve(f, g, method="icgn", precision=_native.PREC_COMP_F32, max_iterations=max_iterations,
+                    tolerance=tolerance, max_norm=max_norm)
+        eng.sync()
+        out.append(b.results())
+        if reference_every and k % reference_every == 0 and k + 1 < len(frames):
+            f.update(frame)
+            b.reset_warps(np.zeros((n, 12)))
+    return out

@@ -39,6 +39,8 @@ EXPORTED = [
     "dic_image_destroy", "dic_mask_local", "dic_mask_update", "dic_batch_create", "dic_batch_set_p0",
     "dic_batch_solve", "dic_batch_results", "dic_batch_history", "dic_batch_subset_arrays",
     "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
+    "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
+    "dic_frame_graph_destroy",
 ]
 
 
@@ -115,6 +117,13 @@ def load():
         L.dic_batch_subset_arrays.argtypes = [_vp, ctypes.c_int64, _vp, _vp]
         L.dic_batch_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.dic_batch_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
+        L.dic_image_stage.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
+        L.dic_batch_reset_warps.argtypes = [_vp, _vp, ctypes.c_int]
+        L.dic_frame_graph_create.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(SolveParams),
+                                             ctypes.POINTER(_vp)]
+        L.dic_frame_graph_launch.argtypes = [_vp]
+        L.dic_frame_graph_destroy.argtypes = [_vp]
+        L.dic_frame_graph_destroy.restype = None
         L.dic_batch_device_p.argtypes = [_vp, ctypes.POINTER(_vp)]
         L.dic_batch_destroy.argtypes = [_vp]
         L.dic_batch_destroy.restype = None
@@ -233,6 +242,19 @@ class DeviceImage:
                 pixels = np.ascontiguousarray(pixels, dtype=np.float64)
             check(L.dic_image_update(self.handle, ptr(pixels), pt, MEM_HOST))
 
+    def stage(self, pixels=None, device_ptr=None, pix_type=None):
+        """Copy a frame's pixels into the image's device pixel buffer (the
+        source its frame graph prefilters)."""
+        L = load()
+        if device_ptr is not None:
+            check(L.dic_image_stage(self.handle, ctypes.c_void_p(device_ptr), pix_type, MEM_DEVICE))
+        else:
+            pixels = np.ascontiguousarray(pixels)
+            pt = PIX_U8 if pixels.dtype == np.uint8 else PIX_F64
+            if pt == PIX_F64:
+                pixels = np.ascontiguousarray(pixels, dtype=np.float64)
+            check(L.dic_image_stage(self.handle, ptr(pixels), pt, MEM_HOST))
+
     def coef(self):
         h, w = self.shape
         b = self.border
@@ -282,6 +304,11 @@ class Batch:
     def set_schedule(self, win_slack, waves_per_subset):
         check(load().dic_batch_set_schedule(self.handle, int(win_slack), int(waves_per_subset)))
 
+    def reset_warps(self, p):
+        """Set both the initial and the final warps (n x 12)."""
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(self.n, 12))
+        check(load().dic_batch_reset_warps(self.handle, ptr(p), MEM_HOST))
+
     def autotune(self, f_img, g_img, max_iterations=50, tolerance=0.75, max_norm=1e-5):
         """Time the batched engine's candidate schedules per size class and
         keep the fastest (results are unspecified until the next solve)."""
@@ -323,6 +350,32 @@ class Batch:
         try:
             if getattr(self, "handle", None):
                 load().dic_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class FrameGraph:
+    """The per-frame steps of an incremental sequence as one HIP graph
+    (dic_frame_graph_*): prefilter of the staged target frame, warm start
+    p0 := p (if asked), batched ICGN solve.  Creating it processes the frame
+    staged at that moment."""
+
+    def __init__(self, batch, f_img, g_img, warm_start=True, max_iterations=50, tolerance=0.75, max_norm=1e-5):
+        self.batch, self.f_img, self.g_img = batch, f_img, g_img  # the graph holds their device buffers
+        prm = SolveParams(METHOD["icgn"], PREC_COMP_F32, int(max_iterations), 0, float(tolerance), float(max_norm))
+        h = _vp()
+        check(load().dic_frame_graph_create(batch.handle, f_img.handle, g_img.handle, 1 if warm_start else 0,
+                                            ctypes.byref(prm), ctypes.byref(h)))
+        self.handle = h
+
+    def launch(self):
+        check(load().dic_frame_graph_launch(self.handle))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                load().dic_frame_graph_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass

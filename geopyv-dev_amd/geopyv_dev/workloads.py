@@ -323,3 +323,20 @@ def shear_band_pair(seed=2, shape=(3000, 4096), n_speckles=150000, n_subsets=150
     p0[:, 1] = np.rint(truth[:, 1])
     return dict(ref=sp.reference(2.0), tar=sp.target(field, 2.0), field=field, coords=coords, radii=r,
                 order=2, p0=p0, truth=truth)
+
+
+def sequence_frames(seed=3, shape=(2048, 2048), n_frames=200, n_speckles=80000, sigma=2.0, slip_max=8.0):
+    """Frames of an incremental sequence: one speckle pattern (σ = 2 px)
+    under a shear band whose slip grows linearly from 0 (frame 0) to
+    slip_max (last frame); grey-level noise σ = 2 drawn per frame from
+    seed(frame).  Returns (frames, fields); frames are 8-bit arrays."""
+    h, w = shape
+    sp = Speckle(shape, n_speckles, sigma, seed)
+    frames, fields = [], []
+    for k in range(n_frames):
+        slip = slip_max * k / max(1, n_frames - 1)
+        field = ShearBand(((w - 1) / 2.0, (h - 1) / 2.0), slip=slip)
+        rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, 1000 + k])))
+        frames.append(sp.target(field, 2.0, rng=rng))
+        fields.append(field)
+    return frames, fields

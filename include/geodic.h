@@ -103,6 +103,12 @@ int dic_image_create(dic_engine* e, const void* pixels, int pix_type, int mem, i
 /* New frame of the same shape into an existing image (Sequence::solve's
  * per-frame Image::from_file, sequence.rs:914): reuses its device buffers. */
 int dic_image_update(dic_image* img, const void* pixels, int pix_type, int mem);
+/* Copies a frame's pixels (host or device memory, the image's shape) into
+ * the image's own device pixel buffer, the source a frame graph prefilters;
+ * the pixel type is fixed by the first call.  Host copies complete before
+ * return.  Per-frame ingest of sequence.rs:693-694,914 (the decode stays on
+ * the host). */
+int dic_image_stage(dic_image* img, const void* pixels, int pix_type, int mem);
 /* Copies C back to the host: (h+2b)*(w+2b) doubles (Image.coef, image.rs:51). */
 int dic_image_coef(const dic_image* img, double* out);
 int dic_image_info(const dic_image* img, int64_t* h, int64_t* w, int* border);
@@ -153,6 +159,27 @@ int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_subset);
  * batch afterwards are unspecified until the next solve.  No reference
  * counterpart (the reference has no device schedule). */
 int dic_batch_autotune(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
+/* ---- frame graphs: the per-frame steps of an incremental sequence
+ *      (sequence.rs:624-968, one target frame) captured once as a HIP graph
+ *      and replayed per frame: prefilter of g_img from its staged pixels,
+ *      then (warm_start != 0) p0 := the batch's final warps, then the batched
+ *      ICGN solve with prm (DIC_PREC_COMP_F32, no history).  Stage each new
+ *      frame with dic_image_stage before the launch; updating f_img in place
+ *      (dic_image_update, a reference update) keeps the graph valid.  The
+ *      graph holds the batch's and images' device buffers: destroy it first.
+ *      No reference counterpart (the reference loops on the host). ---- */
+typedef struct dic_frame_graph dic_frame_graph;
+/* Sets both the initial and the final warps (n_subsets x 12), so that the
+ * next warm-started frame starts from p (e.g. zeros after a reference
+ * update). */
+int dic_batch_reset_warps(dic_batch* b, const double* p, int mem);
+/* Creating a graph processes the staged frame once, eagerly (with the p0 the
+ * batch holds), then records the graph without running it. */
+int dic_frame_graph_create(dic_batch* b, const dic_image* f_img, dic_image* g_img, int warm_start,
+                           const dic_solve_params* prm, dic_frame_graph** out);
+/* Enqueues one frame on the engine's stream (dic_engine_sync waits for it). */
+int dic_frame_graph_launch(dic_frame_graph* g);
+void dic_frame_graph_destroy(dic_frame_graph* g);
 /* Final warps as a device pointer (n_subsets x 12 doubles) for chaining
  * batches without a host round trip. */
 int dic_batch_device_p(dic_batch* b, const double** out);
