@@ -551,7 +551,8 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
     CHECK_ARG(db && f && g && prm, "dic_batch_solve: NULL argument");
     Batch* b = &db->b;
     Engine* e = b->eng;
-    CHECK_ARG(f->im.eng == e && g->im.eng == e, "invalid input: images belong to another engine");
+    CHECK_ARG(f->im.eng->device == e->device && g->im.eng->device == e->device,
+              "invalid input: images live on another device");
     CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
               "invalid input: reference and target images differ in shape or border");
     CHECK_ARG(prm->method == DIC_METHOD_ICGN || prm->method == DIC_METHOD_FAGN, "invalid input: method");
@@ -589,7 +590,8 @@ int dic_batch_autotune(dic_batch* db, const dic_image* f, const dic_image* g, co
     CHECK_ARG(db && f && g && prm, "dic_batch_autotune: NULL argument");
     Batch* b = &db->b;
     Engine* e = b->eng;
-    CHECK_ARG(f->im.eng == e && g->im.eng == e, "invalid input: images belong to another engine");
+    CHECK_ARG(f->im.eng->device == e->device && g->im.eng->device == e->device,
+              "invalid input: images live on another device");
     CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
               "invalid input: reference and target images differ in shape or border");
     CHECK_ARG(prm->max_iterations >= 0 && prm->max_iterations <= 100000, "invalid input: max_iterations");

@@ -325,7 +325,8 @@ def shear_band_pair(seed=2, shape=(3000, 4096), n_speckles=150000, n_subsets=150
                 order=2, p0=p0, truth=truth)
 
 
-def sequence_frames(seed=3, shape=(2048, 2048), n_frames=200, n_speckles=80000, sigma=2.0, slip_max=8.0):
+def sequence_frames(seed=3, shape=(2048, 2048), n_frames=200, n_speckles=80000, sigma=2.0, slip_max=8.0,
+                    progress=None):
     """Frames of an incremental sequence: one speckle pattern (σ = 2 px)
     under a shear band whose slip grows linearly from 0 (frame 0) to
     slip_max (last frame); grey-level noise σ = 2 drawn per frame from
@@ -339,4 +340,6 @@ def sequence_frames(seed=3, shape=(2048, 2048), n_frames=200, n_speckles=80000, 
         rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, 1000 + k])))
         frames.append(sp.target(field, 2.0, rng=rng))
         fields.append(field)
+        if progress is not None:
+            progress(k)
     return frames, fields

@@ -135,7 +135,10 @@ int dic_batch_create(dic_engine* e, int64_t n_subsets, const double* coords, con
 /* Initial warps, n_subsets x 12 doubles (missing entries zero; p0 is
  * truncated or zero-padded to p_len as subset.rs:1135-1136 does). */
 int dic_batch_set_p0(dic_batch* b, const double* p0, int mem);
-/* Queues the solve on the engine stream; returns without waiting. */
+/* Queues the solve on the engine stream; returns without waiting.  The
+ * images may belong to another engine of the same device provided that
+ * engine has been synchronised since it last wrote them (a shared cache of
+ * reference images serving several engines). */
 int dic_batch_solve(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
 /* Waits, then copies n_subsets results to the host. */
 int dic_batch_results(dic_batch* b, dic_subset_result* out);
