@@ -18,6 +18,7 @@ results against it.
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -194,6 +195,73 @@ def run_sequence(args, torch, gp, _native, workloads):
     print(json.dumps(out))
 
 
+def run_serving(args, gp, workloads):
+    """Serving loop: Poisson arrivals of correlation jobs, image sizes drawn
+    from {1024², 2048², 4096x3000} (speckle density 0.012/px², σ = 2 px,
+    affine fields with strains U[-2 %, 2 %]), subset counts log-uniform in
+    1k-150k, radii U{15..40}, warp order 1 or 2; references cached on the
+    device, admission by HBM budget, 4 slots.  Image pairs come from a pool
+    (one reference and two targets per size) so generation stays short."""
+    from geopyv_dev import serving
+
+    rng = np.random.Generator(np.random.PCG64(44))
+    sizes = [(1024, 1024), (2048, 2048), (3000, 4096)]
+    pool = {}
+    t0 = time.time()
+    for si, shape in enumerate(sizes):
+        h, w = shape
+        sp = workloads.Speckle(shape, int(round(0.012 * h * w)), 2.0, 50 + si)
+        ref = sp.reference(2.0)
+        tars = []
+        for k in range(2):
+            g = rng.uniform(-0.02, 0.02, 4)
+            field = workloads.Affine(((w - 1) / 2.0, (h - 1) / 2.0), g[0], g[1], g[2], g[3])
+            tars.append((field, sp.target(field, 2.0, rng=np.random.Generator(np.random.PCG64(200 + 10 * si + k)))))
+        pool[shape] = (ref, tars)
+        log(f"pool {shape} generated ({time.time() - t0:.1f}s)")
+    rates = [float(r) for r in args.rates.split(",")]
+    out = {"metric": "serving loop: ICGN subsets/s and job latency under Poisson arrivals (FP32 engine)",
+           "unit": "subsets/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "serving: jobs of 1024^2 / 2048^2 / 4096x3000 pairs, density 0.012/px^2, affine "
+                                  "strains U[-2%,2%], subsets log-uniform 1k-150k, radii U{15..40}, order 1 or 2; "
+                                  "4 slots, reference cache, HBM budget 64 GiB",
+                      "jobs_per_rate": args.jobs},
+           "rates": {}}
+    for lam in rates:
+        jobs, arrivals, t = [], [], 0.0
+        for j in range(args.jobs):
+            t += rng.exponential(1.0 / lam)
+            shape = sizes[int(rng.integers(0, 3))]
+            h, w = shape
+            ref, tars = pool[shape]
+            field, tar = tars[int(rng.integers(0, 2))]
+            nsub = int(round(math.exp(rng.uniform(math.log(1000), math.log(150000)))))
+            coords = np.column_stack([rng.uniform(46, w - 47, nsub), rng.uniform(46, h - 47, nsub)])
+            coords = np.rint(coords * 4.0) / 4.0
+            truth = field.warp(coords[:, 0], coords[:, 1])
+            p0 = np.zeros((nsub, 12))
+            p0[:, :2] = np.rint(truth[:, :2])
+            jobs.append(serving.Job(j, shape, ref, tar, coords, rng.integers(15, 41, nsub),
+                                    order=int(rng.integers(1, 3)), p0=p0))
+            arrivals.append(t)
+        srv = serving.Server(n_slots=4, hbm_budget=64 << 30)
+        tb = time.perf_counter()
+        res = srv.run(jobs, arrivals=arrivals)
+        wall = time.perf_counter() - tb
+        srv.close()
+        lat = np.array([res[j.job_id][1] for j in jobs])
+        nsub_total = sum(len(j.coords) for j in jobs)
+        conv = np.mean(np.concatenate([res[j.job_id][0]["converged"] for j in jobs]))
+        out["rates"][str(lam)] = {"subsets_per_s": round(nsub_total / wall, 1), "jobs": len(jobs),
+                                  "wall_s": round(wall, 3), "latency_ms": {
+                                      "p50": round(float(np.percentile(lat, 50)), 3),
+                                      "p99": round(float(np.percentile(lat, 99)), 3)},
+                                  "converged_fraction": round(float(conv), 5)}
+        log(f"rate {lam}: {out['rates'][str(lam)]}")
+    out["value"] = out["rates"][str(rates[0])]["subsets_per_s"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,10 +280,13 @@ def main():
                     help="only run a 512 MiB device-to-device copy (known HBM bytes) for counter scaling")
     ap.add_argument("--engine", choices=["both", "f32"], default="both",
                     help="f32: time only the batched engine (for profiler runs)")
-    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band", "sequence"], default="uniform_mesh",
+    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band", "sequence", "serving"],
+                    default="uniform_mesh",
                     help="shear_band: uneven radii, second-order warp; sequence: incremental frames through a "
                          "captured frame graph (each its own line, not the headline)")
     ap.add_argument("--frames", type=int, default=24, help="sequence: frames (the full sequence has 200)")
+    ap.add_argument("--jobs", type=int, default=120, help="serving: jobs per arrival rate (the full stream has 5000)")
+    ap.add_argument("--rates", default="20,50,100", help="serving: Poisson arrival rates, jobs/s")
     ap.add_argument("--no-autotune", action="store_true", help="shear_band: keep the default schedule")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -241,6 +312,9 @@ def main():
         return
     if args.workload == "sequence":
         run_sequence(args, torch, gp, _native, workloads)
+        return
+    if args.workload == "serving":
+        run_serving(args, gp, workloads)
         return
 
     t0 = time.time()

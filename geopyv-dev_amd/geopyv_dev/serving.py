@@ -54,11 +54,14 @@ class Job:
         the target's coefficients, the row-pass buffer of its slot, the
         staged pixels and the batch (per subset: centre, warps, result,
         dispatch order; templates)."""
-        h, w = np.asarray(self.tar).shape
-        hp, wp = h + 2 * self.border, w + 2 * self.border
-        npx = sum(int(np.count_nonzero((np.arange(-r, r + 1)[:, None] ** 2 + np.arange(-r, r + 1)[None, :] ** 2)
-                                       <= r * r)) for r in np.unique(self.radii))
-        return self.image_bytes() + hp * wp * 8 + h * w * 8 + len(self.coords) * 512 + npx * 8 + (1 << 20)
+        if getattr(self, "_bytes", None) is None:
+            h, w = np.asarray(self.tar).shape
+            hp, wp = h + 2 * self.border, w + 2 * self.border
+            ax = np.arange(-int(self.radii.max()), int(self.radii.max()) + 1)
+            d2 = ax[:, None] ** 2 + ax[None, :] ** 2
+            npx = sum(int(np.count_nonzero(d2 <= r * r)) for r in np.unique(self.radii))
+            self._bytes = self.image_bytes() + hp * wp * 8 + h * w * 8 + len(self.coords) * 512 + npx * 8 + (1 << 20)
+        return self._bytes
 
 
 class Server:
