@@ -195,6 +195,100 @@ def run_sequence(args, torch, gp, _native, workloads):
     print(json.dumps(out))
 
 
+def run_mesh(args, torch, gp, _native, workloads):
+    """Mesh mode: the 2048² pair of the headline workload, its 200 x 200
+    grid of nodes (r = 20, 10 px pitch) as a triangular mesh, solved by
+    layer-RG from the seed node nearest the image centre with a zero seed
+    warp: no initial warps are given, the traversal preconditions every node
+    from its solved neighbours (mesh.rs:628-857).  One step = one whole mesh
+    solve (seed, rounds, corrections, strains) with the images resident."""
+    t0 = time.time()
+    w = workloads.uniform_mesh_pair(seed=1)
+    log(f"workload generated in {time.time() - t0:.1f}s")
+    ref, tar, coords, truth = w["ref"], w["tar"], w["coords"], w["truth"]
+    nodes, el = gp.structured_mesh(29.0, 29.0, 2019.0, 2019.0, 10.0)
+    assert np.array_equal(nodes, coords)
+    n = len(nodes)
+    mask = gp.Mask(mask_type="local", shape="circle", size=w["radius"])
+    xy = np.rint(mask.coords).astype(np.int16)
+    tpl_index, tpl_start = np.zeros(n, np.int32), np.array([0, len(xy)], np.int64)
+    eng = gp._engine()
+    dref = torch.from_numpy(ref).cuda()
+    dtar = torch.from_numpy(tar).cuda()
+    torch.cuda.synchronize()
+    f_img = _native.DeviceImage(eng, device_ptr=dref.data_ptr(), shape=ref.shape, pix_type=_native.PIX_U8, border=20)
+    g_img = _native.DeviceImage(eng, device_ptr=dtar.data_ptr(), shape=ref.shape, pix_type=_native.PIX_U8, border=20)
+    eng.sync()
+    seed = [1024.0, 1024.0]
+    # override: corner nodes whose target window leaves the image (up to
+    # 15 px of displacement at the corners) stay short of quality_ok, as
+    # they do in the batch mode; they are counted, not fatal
+    common = dict(subset_order=1, max_iterations=args.max_iterations, tolerance=0.75, max_norm=1e-5,
+                  seed_tolerance=0.9, preconditioning="layer-RG", override=True)
+
+    def solve(precision):
+        t = time.perf_counter()
+        st, msg, r = _native.mesh_solve(eng, f_img, g_img, nodes, el, 1, tpl_index, tpl_start, xy, seed,
+                                        np.zeros(6), precision=precision, **common)
+        dt = time.perf_counter() - t
+        if st:
+            raise RuntimeError(f"mesh solve failed ({st}): {msg}")
+        return r, dt
+
+    for _ in range(max(args.warmup, 1)):
+        solve(_native.PREC_COMP_F32)
+    steps = max(1, min(args.steps, 10))
+    times = []
+    for k in range(steps):
+        r32, dt = solve(_native.PREC_COMP_F32)
+        times.append(dt)
+        log(f"mesh step {k}: {dt * 1e3:.1f} ms, {r32['n_lists']} lists, {r32['n_node_solves']} node solves")
+    r64, dt64 = solve(_native.PREC_PLAIN_F64)
+    log(f"plain FP64 mesh solve: {dt64 * 1e3:.1f} ms")
+    ms = float(np.mean(times)) * 1e3
+    ok = r32["quality_ok"]
+    e32 = np.abs(r32["p"][ok, :2] - truth[ok, :2]).max(axis=1)
+    d32_64 = float(np.abs(r32["p"][:, :2] - r64["p"][:, :2]).max())
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)  # the host share of one GPU
+        Cf, Cg = O.get_c(ref.astype(np.float64), 20, threads=threads), O.get_c(tar.astype(np.float64), 20,
+                                                                                threads=threads)
+        t = time.perf_counter()
+        st, ro = O.mesh_solve(Cf, Cg, ref.shape, 20, nodes, el, tpl_index, mask.coords, tpl_start, seed,
+                              np.zeros(6), preconditioning="layer-RG", threads=threads, workers=threads,
+                              **{k: v for k, v in common.items() if k != "preconditioning"})
+        cpu_s = time.perf_counter() - t
+        log(f"oracle layer-RG mesh solve: {cpu_s:.1f}s on {threads} threads, status {st}")
+        cpu = {"value": round(n / cpu_s, 1), "unit": "nodes/s", "cores": threads, "kind": "port",
+               "sample": f"the whole {n}-node mesh, layer-RG with {threads} workers (the reference's rayon "
+                         "decomposition: one node per task within a colour)",
+               "max_abs_du_gpu_f64_vs_oracle_px": float(np.abs(r64["p"][:, :2] - ro["p"][:, :2]).max()),
+               "max_abs_du_gpu_f32_vs_oracle_px": float(np.abs(r32["p"][:, :2] - ro["p"][:, :2]).max())}
+    out = {
+        "metric": "mesh solve: nodes/s of a layer-RG mesh solve from a seed (FP32 engine)",
+        "value": round(n / (ms * 1e-3), 1), "unit": "nodes/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "mesh: 2048x2048 pair, 50000 speckles sigma 2.5, affine u_x=1% v_y=-0.3% u_y=0.5%, "
+                               "200x200 nodes 10 px pitch as 79202 triangles, circle r=20, order 1, layer-RG "
+                               "from the centre node, zero seed warp",
+                   "nodes": n, "elements": int(len(el)),
+                   "workers": int(torch.cuda.get_device_properties(0).multi_processor_count)},
+        "lists_per_solve": r32["n_lists"], "node_solves_per_solve": r32["n_node_solves"],
+        "nodes_without_quality_ok": r32["n_unsolvable"],
+        "plain_f64_ms": round(dt64 * 1e3, 3), "f32_vs_f64_speedup": round(dt64 * 1e3 / ms, 3),
+        # the largest errors sit at the image edge, where 13-15 px of
+        # displacement carries part of the subset out of the image
+        "abs_du_vs_truth_px_quality_ok_nodes": {q: round(float(np.percentile(e32, v)), 5) for q, v in
+                                                (("p50", 50), ("p99", 99), ("p99.9", 99.9), ("max", 100))}, "max_abs_du_f32_vs_f64_px": d32_64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def run_serving(args, gp, workloads):
     """Serving loop: Poisson arrivals of correlation jobs, image sizes drawn
     from {1024², 2048², 4096x3000} (speckle density 0.012/px², σ = 2 px,
@@ -280,7 +374,7 @@ def main():
                     help="only run a 512 MiB device-to-device copy (known HBM bytes) for counter scaling")
     ap.add_argument("--engine", choices=["both", "f32"], default="both",
                     help="f32: time only the batched engine (for profiler runs)")
-    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band", "sequence", "serving"],
+    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band", "sequence", "serving", "mesh"],
                     default="uniform_mesh",
                     help="shear_band: uneven radii, second-order warp; sequence: incremental frames through a "
                          "captured frame graph (each its own line, not the headline)")
@@ -315,6 +409,9 @@ def main():
         return
     if args.workload == "serving":
         run_serving(args, gp, workloads)
+        return
+    if args.workload == "mesh":
+        run_mesh(args, torch, gp, _native, workloads)
         return
 
     t0 = time.time()

@@ -34,27 +34,6 @@ void dev_free(Engine* e, void* p, size_t bytes) {
     e->bytes_in_use -= (int64_t)bytes;
 }
 
-}  // namespace geodic
-
-using namespace geodic;
-
-struct dic_engine {
-    Engine e;
-};
-struct dic_image {
-    Image im;
-};
-struct dic_batch {
-    Batch b;
-};
-struct dic_frame_graph {
-    Engine* e = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    double* tmp = nullptr;  // the graph's own prefilter row-pass buffer
-    size_t tmp_bytes = 0;
-};
-
 #define CHECK_ARG(cond, msg)                  \
     do {                                      \
         if (!(cond)) {                        \
@@ -62,6 +41,34 @@ struct dic_frame_graph {
             return DIC_ERR_INVALID_INPUT;     \
         }                                     \
     } while (0)
+
+// Argument checks shared by the solve entry points.
+int validate_solve(const Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+    const Engine* e = b->eng;
+    CHECK_ARG(f->eng->device == e->device && g->eng->device == e->device,
+              "invalid input: images live on another device");
+    CHECK_ARG(f->h == g->h && f->w == g->w && f->border == g->border,
+              "invalid input: reference and target images differ in shape or border");
+    CHECK_ARG(prm->method == DIC_METHOD_ICGN || prm->method == DIC_METHOD_FAGN, "invalid input: method");
+    CHECK_ARG(prm->precision == DIC_PREC_PLAIN_F64 || prm->precision == DIC_PREC_COMP_F32, "invalid input: precision");
+    CHECK_ARG(prm->max_iterations >= 0 && prm->max_iterations <= 100000, "invalid input: max_iterations");
+    CHECK_ARG(!(prm->precision == DIC_PREC_COMP_F32 && prm->method == DIC_METHOD_FAGN),
+              "invalid input: the batched engine runs ICGN; use DIC_PREC_PLAIN_F64 for FAGN");
+    CHECK_ARG(b->max_n <= 32767 * 2, "invalid input: template too large");
+    return DIC_OK;
+}
+
+}  // namespace geodic
+
+using namespace geodic;
+
+struct dic_frame_graph {
+    Engine* e = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double* tmp = nullptr;  // the graph's own prefilter row-pass buffer
+    size_t tmp_bytes = 0;
+};
 
 extern "C" {
 
@@ -251,6 +258,11 @@ void dic_image_destroy(dic_image* di) {
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+// Staging layout of a list solve of up to `cap` subsets: the list, its
+// dispatch order (int32 each, 8-byte aligned), then the initial warps.
+static size_t list_ints_bytes(int64_t cap) { return ((size_t)cap * 4 + 7) & ~(size_t)7; }
+static size_t list_io_bytes(int64_t cap) { return 2 * list_ints_bytes(cap) + sizeof(double) * 12 * (size_t)cap; }
+
 static void batch_free(Batch* b) {
     Engine* e = b->eng;
     const int64_t n = b->n;
@@ -268,6 +280,9 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_hist, sizeof(double) * 4 * n * b->hist_cap);
     dev_free(e, b->d_scratch, sizeof(double) * b->scratch_doubles);
     dev_free(e, b->d_order_perm, sizeof(int32_t) * n);
+    dev_free(e, b->d_list_io, list_io_bytes(b->list_cap));
+    dev_free(e, b->d_list_res, sizeof(dic_subset_result) * b->list_cap);
+    if (b->h_list_io) (void)hipHostFree(b->h_list_io);
 }
 
 namespace {
@@ -551,16 +566,8 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
     CHECK_ARG(db && f && g && prm, "dic_batch_solve: NULL argument");
     Batch* b = &db->b;
     Engine* e = b->eng;
-    CHECK_ARG(f->im.eng->device == e->device && g->im.eng->device == e->device,
-              "invalid input: images live on another device");
-    CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
-              "invalid input: reference and target images differ in shape or border");
-    CHECK_ARG(prm->method == DIC_METHOD_ICGN || prm->method == DIC_METHOD_FAGN, "invalid input: method");
-    CHECK_ARG(prm->precision == DIC_PREC_PLAIN_F64 || prm->precision == DIC_PREC_COMP_F32, "invalid input: precision");
-    CHECK_ARG(prm->max_iterations >= 0 && prm->max_iterations <= 100000, "invalid input: max_iterations");
-    CHECK_ARG(!(prm->precision == DIC_PREC_COMP_F32 && prm->method == DIC_METHOD_FAGN),
-              "invalid input: the batched engine runs ICGN; use DIC_PREC_PLAIN_F64 for FAGN");
-    CHECK_ARG(b->max_n <= 32767 * 2, "invalid input: template too large");
+    const int vst = validate_solve(b, &f->im, &g->im, prm);
+    if (vst) return vst;
     DIC_HIP_TRY(hipSetDevice(e->device));
     std::lock_guard<std::mutex> lk(e->mu);
     if (prm->want_history) {
@@ -782,6 +789,134 @@ void dic_frame_graph_destroy(dic_frame_graph* gr) {
     if (gr->graph) (void)hipGraphDestroy(gr->graph);
     dev_free(e, gr->tmp, gr->tmp_bytes);
     delete gr;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// list solves: some subsets of a batch, each from its own initial warp
+// ---------------------------------------------------------------------------
+namespace {
+
+// p0 rows of the list into the batch's initial warps: one thread per value
+__global__ void list_scatter_kernel(const int32_t* __restrict__ idx, const double* __restrict__ src,
+                                    double* __restrict__ p0, int64_t count) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count * 12) return;
+    const int64_t r = k / 12;
+    p0[(int64_t)idx[r] * 12 + (k - r * 12)] = src[k];
+}
+
+// results of the listed subsets, in list order: one thread per 8-byte word
+__global__ void list_gather_kernel(const int32_t* __restrict__ idx, const dic_subset_result* __restrict__ res,
+                                   dic_subset_result* __restrict__ dst, int64_t count) {
+    constexpr int64_t W = sizeof(dic_subset_result) / 8;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count * W) return;
+    const int64_t r = k / W;
+    reinterpret_cast<uint64_t*>(dst)[k] = reinterpret_cast<const uint64_t*>(res + idx[r])[k - r * W];
+}
+static_assert(sizeof(dic_subset_result) % 8 == 0, "result record must be whole 8-byte words");
+
+}  // namespace
+
+namespace geodic {
+
+int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* idx,
+               int64_t count, const double* p0, dic_subset_result* out) {
+    Engine* e = b->eng;
+    if (count == 0) return DIC_OK;
+    // every index in range and listed once (two workgroups must not share a subset)
+    if (b->list_mark.size() != (size_t)b->n) b->list_mark.assign(b->n, 0);
+    if (++b->list_stamp == 0) {
+        std::fill(b->list_mark.begin(), b->list_mark.end(), 0u);
+        b->list_stamp = 1;
+    }
+    for (int64_t k = 0; k < count; ++k) {
+        CHECK_ARG(idx[k] >= 0 && idx[k] < b->n, "invalid input: list index out of range");
+        CHECK_ARG(b->list_mark[idx[k]] != b->list_stamp, "invalid input: a subset is listed twice");
+        b->list_mark[idx[k]] = b->list_stamp;
+    }
+    if (count > b->list_cap) {
+        int64_t cap = 256;
+        while (cap < count) cap *= 2;
+        cap = std::min<int64_t>(cap, b->n);
+        dev_free(e, b->d_list_io, list_io_bytes(b->list_cap));
+        dev_free(e, b->d_list_res, sizeof(dic_subset_result) * b->list_cap);
+        if (b->h_list_io) (void)hipHostFree(b->h_list_io);
+        b->d_list_io = nullptr;
+        b->d_list_res = nullptr;
+        b->h_list_io = nullptr;
+        b->list_cap = 0;
+        int st = dev_alloc(e, (void**)&b->d_list_io, list_io_bytes(cap));
+        if (!st) st = dev_alloc(e, (void**)&b->d_list_res, sizeof(dic_subset_result) * cap);
+        if (st) return st;
+        DIC_HIP_TRY(hipHostMalloc((void**)&b->h_list_io, list_io_bytes(cap) + sizeof(dic_subset_result) * cap));
+        b->list_cap = cap;
+    }
+    const size_t ib = list_ints_bytes(b->list_cap);
+    int32_t* h_idx = reinterpret_cast<int32_t*>(b->h_list_io);
+    int32_t* h_ord = reinterpret_cast<int32_t*>(b->h_list_io + ib);
+    double* h_p0 = reinterpret_cast<double*>(b->h_list_io + 2 * ib);
+    dic_subset_result* h_res = reinterpret_cast<dic_subset_result*>(b->h_list_io + list_io_bytes(b->list_cap));
+    std::memcpy(h_idx, idx, sizeof(int32_t) * count);
+    std::memcpy(h_ord, idx, sizeof(int32_t) * count);
+    // the batched engine's classes need the list sorted by template size
+    std::stable_sort(h_ord, h_ord + count, [&](int32_t a, int32_t c) {
+        const int ta = b->h_tpl_index[a], tc = b->h_tpl_index[c];
+        return b->h_tpl_start[ta + 1] - b->h_tpl_start[ta] > b->h_tpl_start[tc + 1] - b->h_tpl_start[tc];
+    });
+    std::memcpy(h_p0, p0, sizeof(double) * 12 * count);
+    const int32_t* d_idx = reinterpret_cast<const int32_t*>(b->d_list_io);
+    const int32_t* d_ord = reinterpret_cast<const int32_t*>(b->d_list_io + ib);
+    const double* d_src = reinterpret_cast<const double*>(b->d_list_io + 2 * ib);
+    // one upload of the list, its order and the warps (contiguous when the
+    // list fills the buffers, else three pieces)
+    if (count == b->list_cap) {
+        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io, b->h_list_io, list_io_bytes(count), hipMemcpyHostToDevice, e->stream));
+    } else {
+        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io, h_idx, sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream));
+        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io + ib, h_ord, sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream));
+        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io + 2 * ib, h_p0, sizeof(double) * 12 * count, hipMemcpyHostToDevice,
+                                   e->stream));
+    }
+    const int64_t nv = count * 12;
+    list_scatter_kernel<<<(unsigned)((nv + 255) / 256), 256, 0, e->stream>>>(d_idx, d_src, b->d_p0, count);
+    DIC_HIP_TRY(hipGetLastError());
+    int st = prm->precision == DIC_PREC_PLAIN_F64 ? launch_icgn_plain(b, f, g, prm, d_idx, count)
+                                                  : launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count);
+    if (st) return st;
+    const int64_t nw = count * (int64_t)(sizeof(dic_subset_result) / 8);
+    list_gather_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, e->stream>>>(d_idx, b->d_res, b->d_list_res, count);
+    DIC_HIP_TRY(hipGetLastError());
+    DIC_HIP_TRY(hipMemcpyAsync(h_res, b->d_list_res, sizeof(dic_subset_result) * count, hipMemcpyDeviceToHost,
+                               e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    std::memcpy(out, h_res, sizeof(dic_subset_result) * count);
+    b->last_precision = prm->precision;
+    b->last_method = prm->method;
+    b->last_max_it = prm->max_iterations;
+    b->last_history = false;
+    return DIC_OK;
+}
+
+}  // namespace geodic
+
+extern "C" {
+
+int dic_batch_solve_list(dic_batch* db, const dic_image* f, const dic_image* g, const dic_solve_params* prm,
+                         const int32_t* idx, int64_t count, const double* p0, dic_subset_result* out) {
+    CHECK_ARG(db && f && g && prm, "dic_batch_solve_list: NULL argument");
+    CHECK_ARG(count >= 0, "invalid input: negative list length");
+    CHECK_ARG(count == 0 || (idx && p0 && out), "dic_batch_solve_list: NULL array");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    const int vst = validate_solve(b, &f->im, &g->im, prm);
+    if (vst) return vst;
+    CHECK_ARG(!prm->want_history, "invalid input: list solves record no history");
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    return solve_list(b, &f->im, &g->im, prm, idx, count, p0, out);
 }
 
 }  // extern "C"

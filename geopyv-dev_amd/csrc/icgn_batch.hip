@@ -50,7 +50,11 @@ int min_wps(int64_t n) { return n <= 64 * 12 ? 1 : (n <= 128 * 12 ? 2 : (n <= 25
 // Size classes of a batch in dispatch order (the order sorts by size, so
 // each class is a contiguous run): subsets that need the same number of
 // wavefronts launch together; an explicit schedule makes one class.
-std::vector<SizeClass> size_classes(const Batch* b) {
+std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm, int64_t n) {
+    if (!h_perm) {
+        h_perm = b->h_perm.data();
+        n = b->n;
+    }
     std::vector<SizeClass> out;
     auto npx = [&](int64_t s) {
         const int t = b->h_tpl_index[s];
@@ -60,8 +64,8 @@ std::vector<SizeClass> size_classes(const Batch* b) {
         const int32_t* e = &b->h_ext[4 * (size_t)b->h_tpl_index[s]];
         return std::max(e[1] - e[0] + 1, e[3] - e[2] + 1);
     };
-    for (int64_t i = 0; i < b->n;) {
-        const int w = b->waves_per_subset > 0 ? b->waves_per_subset : min_wps(npx(b->h_perm[i]));
+    for (int64_t i = 0; i < n;) {
+        const int w = b->waves_per_subset > 0 ? b->waves_per_subset : min_wps(npx(h_perm[i]));
         SizeClass c;
         c.start = i;
         c.base_wps = w;
@@ -73,9 +77,9 @@ std::vector<SizeClass> size_classes(const Batch* b) {
         }
         c.max_n = 0;
         c.max_side = 0;
-        while (i < b->n && (b->waves_per_subset > 0 || min_wps(npx(b->h_perm[i])) == w)) {
-            c.max_n = std::max<int64_t>(c.max_n, npx(b->h_perm[i]));
-            c.max_side = std::max(c.max_side, side(b->h_perm[i]));
+        while (i < n && (b->waves_per_subset > 0 || min_wps(npx(h_perm[i])) == w)) {
+            c.max_n = std::max<int64_t>(c.max_n, npx(h_perm[i]));
+            c.max_side = std::max(c.max_side, side(h_perm[i]));
             ++i;
         }
         c.count = i - c.start;
@@ -112,13 +116,13 @@ BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_
     return A;
 }
 
-int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t st) {
+int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t st, const int32_t* d_perm = nullptr) {
     if (c.max_n > 64 * 12 * c.wps) {
         set_error("invalid input: subset larger than the wavefronts per subset can hold");
         return DIC_ERR_INVALID_INPUT;
     }
     BatchArgs Ac = A;
-    Ac.perm = b->d_order_perm + c.start;
+    Ac.perm = (d_perm ? d_perm : b->d_order_perm) + c.start;
     Ac.n_sub = c.count;
     Ac.win_slack = c.slack;
     // LDS: the coefficient window (template box, slack, taps, cell rounding
@@ -135,10 +139,11 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
 
 }  // namespace
 
-int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* d_perm,
+                      const int32_t* h_perm, int64_t count) {
     Engine* e = b->eng;
     const BatchArgs A = common_args(b, f, g, prm);
-    const std::vector<SizeClass> cls = size_classes(b);
+    const std::vector<SizeClass> cls = size_classes(b, h_perm, count);
     // the first class runs on the engine's stream, the others on the
     // auxiliary streams, forked from and joined back into it
     const int nstreams = std::min<int>((int)cls.size(), 1 + Engine::kAux);
@@ -148,7 +153,7 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     }
     for (size_t k = 0; k < cls.size(); ++k) {
         hipStream_t st = (k % nstreams == 0) ? e->stream : e->aux[k % nstreams - 1];
-        const int r = launch_class(b, A, cls[k], st);
+        const int r = launch_class(b, A, cls[k], st, d_perm);
         if (r) return r;
     }
     for (int k = 0; k + 1 < nstreams; ++k) {

@@ -25,6 +25,7 @@ struct PlainArgs {
     int border;
     double wmax, hmax;
     int64_t n_sub;
+    const int32_t* list;  // block b solves subset list[b] (every subset in order when null)
     const double* coords;
     const int32_t* orders;
     const int32_t* tpl_index;
@@ -120,8 +121,8 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
     __shared__ int sCtl[4];  // [0] stop flag, [1] status, [2] iterations, [3] converged
     __shared__ double sRes[4];  // last zncc, znssd, norm
 
-    const int64_t s = blockIdx.x;
-    if (s >= A.n_sub) return;
+    if ((int64_t)blockIdx.x >= A.n_sub) return;
+    const int64_t s = A.list ? (int64_t)A.list[blockIdx.x] : (int64_t)blockIdx.x;
     const int lane = threadIdx.x;
     const int t = A.tpl_index[s];
     const int64_t st = A.tpl_start[t];
@@ -475,10 +476,12 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
 
 }  // namespace
 
-int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
+int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* list,
+                      int64_t count) {
     Engine* e = b->eng;
     const int fagn = prm->method == DIC_METHOD_FAGN;
-    const size_t need = (size_t)b->total_px * (fagn ? 6 : 4);
+    const int arrays = fagn ? 6 : 4;
+    const size_t need = (size_t)b->total_px * arrays;
     if (b->scratch_doubles < need) {
         if (b->d_scratch) dev_free(e, b->d_scratch, b->scratch_doubles * sizeof(double));
         b->d_scratch = nullptr;
@@ -487,18 +490,22 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
         if (st) return st;
         b->scratch_doubles = need;
     }
-    // per-subset scratch offsets depend on the per-pixel array count
-    std::vector<int64_t> off(b->n);
-    int64_t acc = 0;
-    for (int64_t s = 0; s < b->n; ++s) {
-        off[s] = acc;
-        const int t = b->h_tpl_index[s];
-        acc += (b->h_tpl_start[t + 1] - b->h_tpl_start[t]) * (fagn ? 6 : 4);
+    // per-subset scratch offsets depend on the per-pixel array count; laid
+    // out again only when that changes
+    if (b->scr_arrays != arrays) {
+        std::vector<int64_t> off(b->n);
+        int64_t acc = 0;
+        for (int64_t s = 0; s < b->n; ++s) {
+            off[s] = acc;
+            const int t = b->h_tpl_index[s];
+            acc += (b->h_tpl_start[t + 1] - b->h_tpl_start[t]) * arrays;
+        }
+        DIC_HIP_TRY(hipMemcpyAsync(b->d_scr_off, off.data(), sizeof(int64_t) * b->n, hipMemcpyHostToDevice, e->stream));
+        // the host vector must outlive the asynchronous copy
+        DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+        b->h_scr_off.swap(off);
+        b->scr_arrays = arrays;
     }
-    DIC_HIP_TRY(hipMemcpyAsync(b->d_scr_off, off.data(), sizeof(int64_t) * b->n, hipMemcpyHostToDevice, e->stream));
-    // the host vector must outlive the asynchronous copy
-    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
-    b->h_scr_off.swap(off);
     PlainArgs A;
     A.fC = f->coef64;
     A.gC = g->coef64;
@@ -506,7 +513,8 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.border = f->border;
     A.wmax = (double)f->w - 1e-9;
     A.hmax = (double)f->h - 1e-9;
-    A.n_sub = b->n;
+    A.n_sub = list ? count : b->n;
+    A.list = list;
     A.coords = b->d_coords;
     A.orders = b->d_orders;
     A.tpl_index = b->d_tpl_index;
@@ -525,7 +533,8 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.max_norm = prm->max_norm;
     // all subsets of one batch share the warp order (checked at creation)
     const int order = b->max_order;
-    const dim3 grid((unsigned)b->n), block(64);
+    if (A.n_sub == 0) return DIC_OK;
+    const dim3 grid((unsigned)A.n_sub), block(64);
     if (order == 0)
         icgn_plain_kernel<2><<<grid, block, 0, e->stream>>>(A);
     else if (order == 1)

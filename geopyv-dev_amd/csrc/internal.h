@@ -97,6 +97,14 @@ struct Batch {
     std::vector<int64_t> h_tpl_start;
     std::vector<int32_t> h_tpl_index;
     std::vector<int64_t> h_scr_off;
+    int scr_arrays = 0;               // plain path: per-pixel arrays the scratch offsets are laid out for
+    // list solves (dic_batch_solve_list), buffers grown on demand to list_cap subsets
+    char* d_list_io = nullptr;                // device: list, its dispatch order, initial warps
+    dic_subset_result* d_list_res = nullptr;  // device: gathered results
+    char* h_list_io = nullptr;                // pinned host: the same layout, then the results
+    int64_t list_cap = 0;
+    std::vector<uint32_t> list_mark;          // duplicate check
+    uint32_t list_stamp = 0;
     int last_precision = -1;
     int last_method = -1;
     int last_max_it = 0;
@@ -108,7 +116,9 @@ struct Batch {
 // engine's, for work captured into graphs
 int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own = nullptr);
 // icgn_plain.hip
-int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
+// list: device array of `count` subset indices to solve (all subsets when null)
+int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
+                      const int32_t* list = nullptr, int64_t count = 0);
 // icgn_batch.hip
 struct SizeClass {
     int64_t start, count;  // run of the dispatch order
@@ -118,13 +128,38 @@ struct SizeClass {
     int64_t max_n;
     int max_side;
 };
-std::vector<SizeClass> size_classes(const Batch* b);
-int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
+// classes of the n subsets h_perm lists (sorted by size); the batch's own
+// dispatch order when h_perm is null
+std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm = nullptr, int64_t n = 0);
+// d_perm/h_perm/count: a dispatch order over some of the batch's subsets
+// (device and host copies); the whole batch in its own order when null
+int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
+                      const int32_t* d_perm = nullptr, const int32_t* h_perm = nullptr, int64_t count = 0);
 // times the candidate schedules of every size class and keeps the fastest
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
+
+// api.hip: solves the listed subsets (each at most once) from p0 (count x
+// 12, host) and copies their results, in list order, to out (host).
+// Synchronous.  The caller holds the engine's lock.
+int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* idx,
+               int64_t count, const double* p0, dic_subset_result* out);
+
+// api.hip: argument checks shared by the solve entry points
+int validate_solve(const Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 
 // device memory accounting
 int dev_alloc(Engine* e, void** p, size_t bytes);
 void dev_free(Engine* e, void* p, size_t bytes);
 
 }  // namespace geodic
+
+// The opaque handles of the C ABI.
+struct dic_engine {
+    geodic::Engine e;
+};
+struct dic_image {
+    geodic::Image im;
+};
+struct dic_batch {
+    geodic::Batch b;
+};

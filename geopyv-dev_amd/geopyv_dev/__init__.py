@@ -16,7 +16,7 @@ import numpy as np
 from . import _native
 from ._native import GeodicError
 
-__all__ = ["Image", "Mask", "Subset", "solve_subsets", "GeodicError"]
+__all__ = ["Image", "Mask", "Subset", "Mesh", "structured_mesh", "quadratic_mesh", "solve_subsets", "GeodicError"]
 
 
 def _engine():
@@ -580,3 +580,6 @@ def solve_sequence(frames, coords, local_mask, subset_order=1, reference_every=2
             f.update(frame)
             b.reset_warps(np.zeros((n, 12)))
     return out
+
+
+from .mesh import Mesh, quadratic_mesh, structured_mesh  # noqa: E402  (uses Image, Mask and the solver helpers above)

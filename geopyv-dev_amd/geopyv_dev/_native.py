@@ -32,6 +32,8 @@ MEM_HOST, MEM_DEVICE = 0, 1
 SHAPE = {"circle": 0, "square": 1, "semicircle": 2}
 METHOD = {"icgn": 0, "fagn": 1}
 PREC_PLAIN_F64, PREC_COMP_F32 = 0, 1
+PRECOND = {"RG": 0, "layer-RG": 1}
+ERR_MESH_GENERATION = 4
 
 EXPORTED = [
     "dic_engine_create", "dic_engine_destroy", "dic_engine_sync", "dic_engine_timing", "dic_last_error",
@@ -40,7 +42,7 @@ EXPORTED = [
     "dic_batch_solve", "dic_batch_results", "dic_batch_history", "dic_batch_subset_arrays",
     "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
     "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
-    "dic_frame_graph_destroy",
+    "dic_frame_graph_destroy", "dic_batch_solve_list", "dic_mesh_solve", "dic_mesh_element_strains",
 ]
 
 
@@ -52,6 +54,31 @@ class SolveParams(ctypes.Structure):
         ("want_history", ctypes.c_int32),
         ("tolerance", ctypes.c_double),
         ("max_norm", ctypes.c_double),
+    ]
+
+
+class MeshParams(ctypes.Structure):
+    _fields_ = [
+        ("solve", SolveParams),
+        ("subset_order", ctypes.c_int32),
+        ("preconditioning", ctypes.c_int32),
+        ("override_active", ctypes.c_int32),
+        ("batch_factor", ctypes.c_int32),
+        ("workers", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("root_rel_eps", ctypes.c_double),
+        ("seed_coord", ctypes.c_double * 2),
+        ("seed_warp", ctypes.c_double * 12),
+        ("seed_tolerance", ctypes.c_double),
+    ]
+
+
+class MeshOutput(ctypes.Structure):
+    _fields_ = [
+        ("p", ctypes.c_void_p), ("c_zncc", ctypes.c_void_p), ("iterations", ctypes.c_void_p),
+        ("norms", ctypes.c_void_p), ("quality_ok", ctypes.c_void_p), ("warps", ctypes.c_void_p),
+        ("seed_node", ctypes.c_int64), ("n_node_solves", ctypes.c_int64), ("n_lists", ctypes.c_int64),
+        ("n_unsolvable", ctypes.c_int64),
     ]
 
 
@@ -127,6 +154,12 @@ def load():
         L.dic_batch_device_p.argtypes = [_vp, ctypes.POINTER(_vp)]
         L.dic_batch_destroy.argtypes = [_vp]
         L.dic_batch_destroy.restype = None
+        L.dic_batch_solve_list.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams), _vp, ctypes.c_int64, _vp,
+                                           _vp]
+        L.dic_mesh_solve.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
+                                     ctypes.c_int32, _vp, _vp, ctypes.POINTER(MeshParams),
+                                     ctypes.POINTER(MeshOutput)]
+        L.dic_mesh_element_strains.argtypes = [ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp]
         _lib = L
         return L
 
@@ -322,6 +355,18 @@ class Batch:
         self.max_iterations = int(max_iterations)
         check(load().dic_batch_solve(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm)))
 
+    def solve_list(self, f_img, g_img, idx, p0, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50,
+                   tolerance=0.75, max_norm=1e-5):
+        """Solve only the subsets idx (each once) from p0 (len(idx) x 12);
+        returns their results in list order."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32).ravel()
+        p0 = np.ascontiguousarray(np.asarray(p0, dtype=np.float64).reshape(len(idx), 12))
+        out = np.zeros(len(idx), dtype=RESULT_DTYPE)
+        prm = SolveParams(METHOD[method], int(precision), int(max_iterations), 0, float(tolerance), float(max_norm))
+        check(load().dic_batch_solve_list(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm), ptr(idx),
+                                          len(idx), ptr(p0), ptr(out)))
+        return out
+
     def results(self):
         out = np.zeros(self.n, dtype=RESULT_DTYPE)
         check(load().dic_batch_results(self.handle, ptr(out)))
@@ -379,3 +424,63 @@ class FrameGraph:
                 self.handle = None
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# mesh solve
+# ---------------------------------------------------------------------------
+def mesh_element_strains(nodes, elements, displacements, mesh_order):
+    """element_strains (host): warps (n_elements x 12)."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 2)
+    el = np.ascontiguousarray(elements, dtype=np.int64)
+    d = np.ascontiguousarray(displacements, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((len(el), 12))
+    check(load().dic_mesh_element_strains(len(nodes), ptr(nodes), len(el), int(mesh_order), ptr(el), ptr(d),
+                                          ptr(out)))
+    return out
+
+
+def mesh_solve(engine, f_img, g_img, nodes, elements, mesh_order, tpl_index, tpl_start, tpl_xy, seed_coord,
+               seed_warp=None, subset_order=1, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50,
+               tolerance=0.75, max_norm=1e-5, seed_tolerance=0.9, preconditioning="RG", override=False,
+               batch_factor=4, root_rel_eps=0.02, workers=0):
+    """dic_mesh_solve.  Returns (status, message, outputs dict); outputs
+    are filled when the propagation completed (status 0, or the quality
+    gate, a degenerate or a folded element)."""
+    L = load()
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 2)
+    el = np.ascontiguousarray(elements, dtype=np.int64)
+    tpl_index = np.ascontiguousarray(tpl_index, dtype=np.int32)
+    tpl_start = np.ascontiguousarray(tpl_start, dtype=np.int64)
+    tpl_xy = np.ascontiguousarray(tpl_xy, dtype=np.int16).reshape(-1, 2)
+    n, ne = len(nodes), len(el)
+    prm = MeshParams()
+    prm.solve = SolveParams(METHOD[method], int(precision), int(max_iterations), 0, float(tolerance),
+                            float(max_norm))
+    prm.subset_order = int(subset_order)
+    prm.preconditioning = PRECOND[preconditioning]
+    prm.override_active = 1 if override else 0
+    prm.batch_factor = int(batch_factor)
+    prm.workers = int(workers)
+    prm.root_rel_eps = float(root_rel_eps)
+    prm.seed_coord[0], prm.seed_coord[1] = float(seed_coord[0]), float(seed_coord[1])
+    sw = np.zeros(12)
+    if seed_warp is not None:
+        v = np.asarray(seed_warp, dtype=np.float64).ravel()[:12]
+        sw[:len(v)] = v
+    for k in range(12):
+        prm.seed_warp[k] = sw[k]
+    prm.seed_tolerance = float(seed_tolerance)
+    res = dict(p=np.zeros((n, 12)), c_zncc=np.zeros(n), iterations=np.zeros(n, dtype=np.int32), norms=np.zeros(n),
+               quality_ok=np.zeros(n, dtype=np.int32), warps=np.zeros((ne, 12)))
+    out = MeshOutput(res["p"].ctypes.data, res["c_zncc"].ctypes.data, res["iterations"].ctypes.data,
+                     res["norms"].ctypes.data, res["quality_ok"].ctypes.data, res["warps"].ctypes.data, -1, 0, 0, 0)
+    st = L.dic_mesh_solve(engine.handle, f_img.handle, g_img.handle, n, ptr(nodes), ne, int(mesh_order), ptr(el),
+                          ptr(tpl_index), len(tpl_start) - 1, ptr(tpl_start), ptr(tpl_xy), ctypes.byref(prm),
+                          ctypes.byref(out))
+    msg = L.dic_last_error().decode(errors="replace") if st else ""
+    res["p"] = res["p"][:, :6 * int(subset_order)]
+    res["quality_ok"] = res["quality_ok"].astype(bool)
+    res.update(seed_node=int(out.seed_node), n_node_solves=int(out.n_node_solves), n_lists=int(out.n_lists),
+               n_unsolvable=int(out.n_unsolvable))
+    return st, msg, res

@@ -162,6 +162,16 @@ int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_subset);
  * batch afterwards are unspecified until the next solve.  No reference
  * counterpart (the reference has no device schedule). */
 int dic_batch_autotune(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
+/* Solves only the listed subsets of a batch (each index at most once), each
+ * from its row of p0 (count x 12, host; missing entries zero), and copies
+ * their results to out (count records, host) in list order; the other
+ * subsets keep their warps and results.  Waits for the device.  No history
+ * (prm->want_history must be 0).  This is one round of node solves of the
+ * reference's mesh: the per-node Subset::new + solve_icgn_result calls of
+ * a layer-RG colour (mesh.rs:1224-1233) or of one serial RG step
+ * (mesh.rs:1133-1140). */
+int dic_batch_solve_list(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm,
+                         const int32_t* idx, int64_t count, const double* p0, dic_subset_result* out);
 /* ---- frame graphs: the per-frame steps of an incremental sequence
  *      (sequence.rs:624-968, one target frame) captured once as a HIP graph
  *      and replayed per frame: prefilter of g_img from its staged pixels,
@@ -187,6 +197,56 @@ void dic_frame_graph_destroy(dic_frame_graph* g);
  * batches without a host round trip. */
 int dic_batch_device_p(dic_batch* b, const double** out);
 void dic_batch_destroy(dic_batch* b);
+
+/* ---- mesh solve: Mesh::solve (mesh.rs:609-857) for uniform masking and
+ *      deformable texture, on a mesh given as nodes and elements ----
+ * nodes (n_nodes x 2, [x, y]); elements (n_elements x 3 for mesh_order 1,
+ * x 6 for mesh_order 2: corners then midsides m01, m12, m20), int64 node
+ * indices.  Node i's pixel list is template tpl_index[i] (as for
+ * dic_batch_create): the local mask clipped by the region of interest, as
+ * subset_at builds it (mesh.rs:575-593). */
+enum { DIC_PRECOND_RG = 0, DIC_PRECOND_LAYER_RG = 1 };  /* Preconditioning, mesh.rs:299-303 */
+typedef struct {
+    dic_solve_params solve;   /* SolveConfig (mesh.rs:222-248): method, max_iterations, tolerance,
+                                 max_norm, plus the engine precision; want_history must be 0 */
+    int32_t subset_order;     /* SolveConfig.subset_order: 1 or 2 */
+    int32_t preconditioning;  /* SolveConfig.preconditioning: DIC_PRECOND_* */
+    int32_t override_active;  /* SolveConfig.override_active: keep a mesh with nodes short of quality_ok */
+    int32_t batch_factor;     /* LayerRgConfig.batch_factor (mesh.rs:339-356) */
+    int32_t workers;          /* LayerRgConfig.max_workers; 0 = the device's compute units */
+    int32_t reserved;
+    double root_rel_eps;      /* LayerRgConfig.root_rel_eps */
+    double seed_coord[2];     /* SeedConfig.coord (mesh.rs:269-278) */
+    double seed_warp[12];     /* SeedConfig.warp, zero padded to 12 */
+    double seed_tolerance;    /* SeedConfig.tolerance */
+} dic_mesh_params;
+/* Caller-owned outputs (MeshSolution, mesh.rs:54-113). */
+typedef struct {
+    double* p;               /* n_nodes x 12 warps (p_len = 6 * subset_order used) */
+    double* c_zncc;          /* n_nodes, clamped at 0 (mesh.rs:2396) */
+    int32_t* iterations;     /* n_nodes */
+    double* norms;           /* n_nodes: last convergence norm, 0 without iterations */
+    int32_t* quality_ok;     /* n_nodes */
+    double* warps;           /* n_elements x 12: element_strains (mesh.rs:1552-1665) */
+    int64_t seed_node;       /* find_seed_node (mesh.rs:1682-1697) */
+    int64_t n_node_solves;   /* subset solves run, retries and corrections included */
+    int64_t n_lists;         /* device list solves launched */
+    int64_t n_unsolvable;    /* nodes without quality_ok after corrections */
+} dic_mesh_output;
+/* Seed, RG or layer-RG propagation, corrections, the quality gate, element
+ * strains and the compatibility check.  Returns DIC_ERR_INVALID_INPUT for
+ * an unsolvable mesh (unless override_active) or a folded element
+ * (det F <= 0), DIC_ERR_MESH_GENERATION for a degenerate element, a subset
+ * status for a failed node solve (as the reference's `?` does); the
+ * outputs are filled whenever the propagation itself completed. */
+int dic_mesh_solve(dic_engine* e, const dic_image* f_img, const dic_image* g_img, int64_t n_nodes,
+                   const double* nodes, int64_t n_elements, int32_t mesh_order, const int64_t* elements,
+                   const int32_t* tpl_index, int32_t n_templates, const int64_t* tpl_start, const int16_t* tpl_xy,
+                   const dic_mesh_params* prm, dic_mesh_output* out);
+/* element_strains (mesh.rs:1552-1665) alone, host-side: warps
+ * (n_elements x 12) from node displacements (n_nodes x 2). */
+int dic_mesh_element_strains(int64_t n_nodes, const double* nodes, int64_t n_elements, int32_t mesh_order,
+                             const int64_t* elements, const double* displacements, double* warps);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@
 #include <omp.h>
 #endif
 
+#include "oracle_types.h"
+
 typedef std::complex<double> cplx;
 
 extern "C" {
@@ -748,16 +750,8 @@ static int subset_new(const Img& fim, const double* coord, const double* offsets
     return ORC_OK;
 }
 
-// Result record shared with the Python binding (field order fixed).
-struct OrcResult {
-    double p[12];
-    double c_zncc, c_znssd;
-    double norm;       // last convergence norm (NaN if no iteration)
-    double eta_u, eta_v;
-    double f_m, delta_f, sssig, sigma;
-    int32_t iterations, converged, quality_ok, status;
-    int32_t n_px, order;
-};
+// OrcResult (oracle_types.h) is the result record shared with the Python
+// binding (field order fixed).
 
 // solve_icgn_result (subset.rs:1117-1219) and solve_fagn_result
 // (subset.rs:1228-1334), deformable texture.  history (max_it x 4) optional.

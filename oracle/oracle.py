@@ -1,4 +1,5 @@
-"""ctypes binding of the FP64 CPU oracle (oracle/dic_oracle.cpp).
+"""ctypes binding of the FP64 CPU oracle (oracle/dic_oracle.cpp and
+oracle/mesh_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of bench.py may import this module, and only as the
@@ -23,8 +24,8 @@ mask_local             reference Rust and Python golden counts and first /
                        last coordinates (masks.rs:274-371,
                        test_templates.py); Gauss circle counts
 mask_update            full/zero masks (masks.rs:373-389,
-                       test_templates.py:178-199); committed known-answer
-                       vectors for edge windows (tests/golden)
+                       test_templates.py:178-199); hand-derived known-answer
+                       vectors for edge windows (tests/test_oracle.py)
 inverse                closed-form identities (A A^-1 = I), numpy.linalg
 compose_icgn           golden vector (subset.rs:1790-1810); identity, order
                        0, translation, undo and third-order tests
@@ -344,3 +345,177 @@ def batch_solve(fcoef, gcoef, shape, border, coords, tpl_index, tpl_xy, tpl_star
 
 def wall():
     return lib().orc_wall_seconds()
+
+
+# ---------------------------------------------------------------------------
+# mesh orchestration (mesh_oracle.cpp; mesh.rs)
+# ---------------------------------------------------------------------------
+class OrcMeshParams(ctypes.Structure):
+    _fields_ = [
+        ("subset_order", ctypes.c_int32), ("method", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
+        ("preconditioning", ctypes.c_int32), ("override_active", ctypes.c_int32), ("batch_factor", ctypes.c_int32),
+        ("workers", ctypes.c_int32), ("threads", ctypes.c_int32),
+        ("tolerance", ctypes.c_double), ("max_norm", ctypes.c_double), ("root_rel_eps", ctypes.c_double),
+        ("seed_tolerance", ctypes.c_double), ("seed_coord", ctypes.c_double * 2), ("seed_warp", ctypes.c_double * 12),
+    ]
+
+
+class OrcMeshOut(ctypes.Structure):
+    _fields_ = [
+        ("p", _dp), ("c_zncc", _dp), ("iterations", _i32p), ("norms", _dp), ("quality_ok", _i32p), ("warps", _dp),
+        ("seed_node", ctypes.c_int64), ("n_solves", ctypes.c_int64), ("n_bad", ctypes.c_int64),
+    ]
+
+
+def _el(elements):
+    el = np.ascontiguousarray(elements, dtype=np.int64)
+    return el, el.shape[0], el.shape[1], (1 if el.shape[1] == 3 else 2)
+
+
+def mesh_connectivity(elements, idx, full, mesh_order=None):
+    el, ne, cols, order = _el(elements)
+    order = order if mesh_order is None else mesh_order
+    out = np.zeros(cols * ne + 1, dtype=np.int64)
+    k = lib().orc_mesh_connectivity(el.ctypes.data_as(_i64p), ctypes.c_int64(ne), cols, order, ctypes.c_int64(idx),
+                                    int(bool(full)), out.ctypes.data_as(_i64p))
+    return list(out[:k])
+
+
+def mesh_find_seed(nodes, coord):
+    nodes = _c64(nodes)
+    lib().orc_mesh_find_seed.restype = ctypes.c_int64
+    return int(lib().orc_mesh_find_seed(_d(nodes), ctypes.c_int64(len(nodes)), ctypes.c_double(coord[0]),
+                                        ctypes.c_double(coord[1])))
+
+
+def mesh_corr_1d(vals):
+    v = _c64(vals)
+    out = np.zeros(len(v) + 1, dtype=np.int64)
+    lib().orc_mesh_corr_1d.restype = ctypes.c_int64
+    k = lib().orc_mesh_corr_1d(_d(v), ctypes.c_int64(len(v)), out.ctypes.data_as(_i64p))
+    return list(out[:k])
+
+
+def mesh_flow_calc(elements, n_nodes, idx, disp, exclude=(), override=None):
+    el, ne, cols, order = _el(elements)
+    disp = _c64(disp)
+    ex = np.zeros(n_nodes, dtype=np.uint8)
+    ex[list(exclude)] = 1
+    ov = None if override is None else _c64(override)
+    L = lib()
+    L.orc_mesh_flow_calc.restype = ctypes.c_double
+    return L.orc_mesh_flow_calc(el.ctypes.data_as(_i64p), ctypes.c_int64(ne), cols, order, ctypes.c_int64(n_nodes),
+                                ctypes.c_int64(idx), _d(disp), ex.ctypes.data_as(_u8p),
+                                None if ov is None else _d(ov))
+
+
+def mesh_r_stats(elements, disp):
+    el, ne, cols, order = _el(elements)
+    disp = _c64(disp)
+    n = len(disp)
+    r = np.zeros(n)
+    flag = np.zeros(n, dtype=np.uint8)
+    lib().orc_mesh_r_stats(el.ctypes.data_as(_i64p), ctypes.c_int64(ne), cols, order, ctypes.c_int64(n), _d(disp),
+                           _d(r), flag.ctypes.data_as(_u8p))
+    return r, flag.astype(bool)
+
+
+def mesh_project_warp(p, dx, dy):
+    p = _c64(p)
+    out = np.zeros(len(p))
+    lib().orc_mesh_project_warp(_d(p), len(p), ctypes.c_double(dx), ctypes.c_double(dy), _d(out))
+    return out
+
+
+def mesh_affine_field_fit(nodes, target, trusted, p):
+    nodes, p = _c64(nodes), _c64(p)
+    t = np.ascontiguousarray(trusted, dtype=np.int64)
+    tg = _c64(target)
+    out = np.zeros(p.shape[1])
+    ok = lib().orc_mesh_affine_field_fit(_d(nodes), _d(tg), t.ctypes.data_as(_i64p), ctypes.c_int64(len(t)), _d(p),
+                                         p.shape[1], _d(out))
+    return out if ok else None
+
+
+def mesh_precondition_warp(nodes, target, trusted, c_zncc, p):
+    nodes, p, cz = _c64(nodes), _c64(p), _c64(c_zncc)
+    t = np.ascontiguousarray(trusted, dtype=np.int64)
+    tg = _c64(target)
+    out = np.zeros(p.shape[1])
+    lib().orc_mesh_precondition_warp(_d(nodes), _d(tg), t.ctypes.data_as(_i64p), ctypes.c_int64(len(t)), _d(cz),
+                                     _d(p), p.shape[1], _d(out))
+    return out
+
+
+def mesh_greedy_colour(elements, n_nodes, candidates):
+    el, ne, cols, order = _el(elements)
+    c = np.ascontiguousarray(candidates, dtype=np.int64)
+    out = np.zeros(len(c), dtype=np.int64)
+    lib().orc_mesh_greedy_colour(el.ctypes.data_as(_i64p), ctypes.c_int64(ne), cols, order, ctypes.c_int64(n_nodes),
+                                 c.ctypes.data_as(_i64p), ctypes.c_int64(len(c)), out.ctypes.data_as(_i64p))
+    return list(out)
+
+
+def mesh_element_area(nodes, elements):
+    el, ne, cols, _ = _el(elements)
+    nodes = _c64(nodes)
+    out = np.zeros(ne)
+    lib().orc_mesh_element_area(_d(nodes), el.ctypes.data_as(_i64p), ctypes.c_int64(ne), cols, _d(out))
+    return out
+
+
+def mesh_element_strains(nodes, elements, disp, mesh_order=None):
+    el, ne, cols, order = _el(elements)
+    order = order if mesh_order is None else mesh_order
+    nodes, disp = _c64(nodes), _c64(disp)
+    out = np.zeros((ne, 12))
+    st = lib().orc_mesh_element_strains(_d(nodes), el.ctypes.data_as(_i64p), ctypes.c_int64(ne), order, _d(disp),
+                                        _d(out))
+    return st, out
+
+
+def mesh_solve(fcoef, gcoef, shape, border, nodes, elements, tpl_index, tpl_xy, tpl_start, seed_coord,
+               seed_warp=None, subset_order=1, tolerance=0.75, seed_tolerance=0.9, max_norm=1e-5, max_iterations=50,
+               method="icgn", preconditioning="RG", override=False, batch_factor=4, root_rel_eps=0.02, workers=0,
+               threads=0):
+    """Mesh::solve_impl restated (uniform masking, deformable texture).
+    Returns (status, dict of per-node and per-element arrays)."""
+    fcoef, gcoef, nodes = _c64(fcoef), _c64(gcoef), _c64(nodes)
+    el, ne, cols, order = _el(elements)
+    tpl_index = np.ascontiguousarray(tpl_index, dtype=np.int32)
+    tpl_xy = _c64(tpl_xy)
+    tpl_start = np.ascontiguousarray(tpl_start, dtype=np.int64)
+    n = len(nodes)
+    prm = OrcMeshParams()
+    prm.subset_order = subset_order
+    prm.method = 0 if method == "icgn" else 1
+    prm.max_iterations = max_iterations
+    prm.preconditioning = 0 if preconditioning == "RG" else 1
+    prm.override_active = int(bool(override))
+    prm.batch_factor = batch_factor
+    prm.workers = workers
+    prm.threads = threads
+    prm.tolerance = tolerance
+    prm.max_norm = max_norm
+    prm.root_rel_eps = root_rel_eps
+    prm.seed_tolerance = seed_tolerance
+    prm.seed_coord[0], prm.seed_coord[1] = float(seed_coord[0]), float(seed_coord[1])
+    sw = np.zeros(12)
+    if seed_warp is not None:
+        v = np.asarray(seed_warp, dtype=np.float64).ravel()[:12]
+        sw[:len(v)] = v
+    for k in range(12):
+        prm.seed_warp[k] = sw[k]
+    res = dict(p=np.zeros((n, 12)), c_zncc=np.zeros(n), iterations=np.zeros(n, dtype=np.int32), norms=np.zeros(n),
+               quality_ok=np.zeros(n, dtype=np.int32), warps=np.zeros((ne, 12)))
+    out = OrcMeshOut(_d(res["p"]), _d(res["c_zncc"]), res["iterations"].ctypes.data_as(_i32p), _d(res["norms"]),
+                     res["quality_ok"].ctypes.data_as(_i32p), _d(res["warps"]), 0, 0, 0)
+    st = lib().orc_mesh_solve(_d(fcoef), _d(gcoef), shape[0], shape[1], border, _d(nodes), ctypes.c_int64(n),
+                              el.ctypes.data_as(_i64p), ctypes.c_int64(ne), order,
+                              tpl_index.ctypes.data_as(_i32p), _d(tpl_xy), tpl_start.ctypes.data_as(_i64p),
+                              ctypes.byref(prm), ctypes.byref(out))
+    m = 6 * subset_order
+    res["p"] = res["p"][:, :m]
+    res["quality_ok"] = res["quality_ok"].astype(bool)
+    res.update(seed_node=int(out.seed_node), n_solves=int(out.n_solves), n_bad=int(out.n_bad))
+    return st, res
