@@ -242,7 +242,8 @@ def run_mesh(args, torch, gp, _native, workloads):
     for k in range(steps):
         r32, dt = solve(_native.PREC_COMP_F32)
         times.append(dt)
-        log(f"mesh step {k}: {dt * 1e3:.1f} ms, {r32['n_lists']} lists, {r32['n_node_solves']} node solves")
+        log(f"mesh step {k}: {dt * 1e3:.1f} ms, {r32['n_lists']} lists ({r32['list_seconds'] * 1e3:.1f} ms in list "
+            f"solves), {r32['n_node_solves']} node solves")
     r64, dt64 = solve(_native.PREC_PLAIN_F64)
     log(f"plain FP64 mesh solve: {dt64 * 1e3:.1f} ms")
     ms = float(np.mean(times)) * 1e3
@@ -278,6 +279,7 @@ def run_mesh(args, torch, gp, _native, workloads):
                    "nodes": n, "elements": int(len(el)),
                    "workers": int(torch.cuda.get_device_properties(0).multi_processor_count)},
         "lists_per_solve": r32["n_lists"], "node_solves_per_solve": r32["n_node_solves"],
+        "list_solve_ms": round(r32["list_seconds"] * 1e3, 3),
         "nodes_without_quality_ok": r32["n_unsolvable"],
         "plain_f64_ms": round(dt64 * 1e3, 3), "f32_vs_f64_speedup": round(dt64 * 1e3 / ms, 3),
         # the largest errors sit at the image edge, where 13-15 px of

@@ -259,7 +259,8 @@ void dic_image_destroy(dic_image* di) {
 // batches
 // ---------------------------------------------------------------------------
 // Staging layout of a list solve of up to `cap` subsets: the list, its
-// dispatch order (int32 each, 8-byte aligned), then the initial warps.
+// dispatch order (int32 each, 8-byte aligned), the initial warps, then (at
+// list_io_bytes) the results.
 static size_t list_ints_bytes(int64_t cap) { return ((size_t)cap * 4 + 7) & ~(size_t)7; }
 static size_t list_io_bytes(int64_t cap) { return 2 * list_ints_bytes(cap) + sizeof(double) * 12 * (size_t)cap; }
 
@@ -280,8 +281,6 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_hist, sizeof(double) * 4 * n * b->hist_cap);
     dev_free(e, b->d_scratch, sizeof(double) * b->scratch_doubles);
     dev_free(e, b->d_order_perm, sizeof(int32_t) * n);
-    dev_free(e, b->d_list_io, list_io_bytes(b->list_cap));
-    dev_free(e, b->d_list_res, sizeof(dic_subset_result) * b->list_cap);
     if (b->h_list_io) (void)hipHostFree(b->h_list_io);
 }
 
@@ -837,28 +836,27 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
         CHECK_ARG(b->list_mark[idx[k]] != b->list_stamp, "invalid input: a subset is listed twice");
         b->list_mark[idx[k]] = b->list_stamp;
     }
+    // staging in pinned, device-mapped host memory: the kernels read the
+    // list and its warps there and write the results there, so a list
+    // costs its launches and one synchronisation, no copies
     if (count > b->list_cap) {
         int64_t cap = 256;
         while (cap < count) cap *= 2;
         cap = std::min<int64_t>(cap, b->n);
-        dev_free(e, b->d_list_io, list_io_bytes(b->list_cap));
-        dev_free(e, b->d_list_res, sizeof(dic_subset_result) * b->list_cap);
         if (b->h_list_io) (void)hipHostFree(b->h_list_io);
-        b->d_list_io = nullptr;
-        b->d_list_res = nullptr;
         b->h_list_io = nullptr;
+        b->d_list_io = nullptr;
         b->list_cap = 0;
-        int st = dev_alloc(e, (void**)&b->d_list_io, list_io_bytes(cap));
-        if (!st) st = dev_alloc(e, (void**)&b->d_list_res, sizeof(dic_subset_result) * cap);
-        if (st) return st;
         DIC_HIP_TRY(hipHostMalloc((void**)&b->h_list_io, list_io_bytes(cap) + sizeof(dic_subset_result) * cap));
+        DIC_HIP_TRY(hipHostGetDevicePointer((void**)&b->d_list_io, b->h_list_io, 0));
         b->list_cap = cap;
     }
     const size_t ib = list_ints_bytes(b->list_cap);
+    const size_t res_off = list_io_bytes(b->list_cap);
     int32_t* h_idx = reinterpret_cast<int32_t*>(b->h_list_io);
     int32_t* h_ord = reinterpret_cast<int32_t*>(b->h_list_io + ib);
     double* h_p0 = reinterpret_cast<double*>(b->h_list_io + 2 * ib);
-    dic_subset_result* h_res = reinterpret_cast<dic_subset_result*>(b->h_list_io + list_io_bytes(b->list_cap));
+    const dic_subset_result* h_res = reinterpret_cast<const dic_subset_result*>(b->h_list_io + res_off);
     std::memcpy(h_idx, idx, sizeof(int32_t) * count);
     std::memcpy(h_ord, idx, sizeof(int32_t) * count);
     // the batched engine's classes need the list sorted by template size
@@ -870,27 +868,16 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
     const int32_t* d_idx = reinterpret_cast<const int32_t*>(b->d_list_io);
     const int32_t* d_ord = reinterpret_cast<const int32_t*>(b->d_list_io + ib);
     const double* d_src = reinterpret_cast<const double*>(b->d_list_io + 2 * ib);
-    // one upload of the list, its order and the warps (contiguous when the
-    // list fills the buffers, else three pieces)
-    if (count == b->list_cap) {
-        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io, b->h_list_io, list_io_bytes(count), hipMemcpyHostToDevice, e->stream));
-    } else {
-        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io, h_idx, sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream));
-        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io + ib, h_ord, sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream));
-        DIC_HIP_TRY(hipMemcpyAsync(b->d_list_io + 2 * ib, h_p0, sizeof(double) * 12 * count, hipMemcpyHostToDevice,
-                                   e->stream));
-    }
+    dic_subset_result* d_res = reinterpret_cast<dic_subset_result*>(b->d_list_io + res_off);
     const int64_t nv = count * 12;
     list_scatter_kernel<<<(unsigned)((nv + 255) / 256), 256, 0, e->stream>>>(d_idx, d_src, b->d_p0, count);
     DIC_HIP_TRY(hipGetLastError());
     int st = prm->precision == DIC_PREC_PLAIN_F64 ? launch_icgn_plain(b, f, g, prm, d_idx, count)
-                                                  : launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count);
+                                                  : launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count, true);
     if (st) return st;
     const int64_t nw = count * (int64_t)(sizeof(dic_subset_result) / 8);
-    list_gather_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, e->stream>>>(d_idx, b->d_res, b->d_list_res, count);
+    list_gather_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, e->stream>>>(d_idx, b->d_res, d_res, count);
     DIC_HIP_TRY(hipGetLastError());
-    DIC_HIP_TRY(hipMemcpyAsync(h_res, b->d_list_res, sizeof(dic_subset_result) * count, hipMemcpyDeviceToHost,
-                               e->stream));
     DIC_HIP_TRY(hipStreamSynchronize(e->stream));
     std::memcpy(out, h_res, sizeof(dic_subset_result) * count);
     b->last_precision = prm->precision;

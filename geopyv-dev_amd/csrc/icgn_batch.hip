@@ -140,10 +140,13 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
 }  // namespace
 
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* d_perm,
-                      const int32_t* h_perm, int64_t count) {
+                      const int32_t* h_perm, int64_t count, bool latency) {
     Engine* e = b->eng;
     const BatchArgs A = common_args(b, f, g, prm);
-    const std::vector<SizeClass> cls = size_classes(b, h_perm, count);
+    std::vector<SizeClass> cls = size_classes(b, h_perm, count);
+    if (latency && b->waves_per_subset == 0)
+        for (SizeClass& c : cls)
+            while (c.wps < 8 && c.count * 2 * c.wps <= 8 * (int64_t)e->num_cus) c.wps *= 2;
     // the first class runs on the engine's stream, the others on the
     // auxiliary streams, forked from and joined back into it
     const int nstreams = std::min<int>((int)cls.size(), 1 + Engine::kAux);

@@ -98,10 +98,11 @@ struct Batch {
     std::vector<int32_t> h_tpl_index;
     std::vector<int64_t> h_scr_off;
     int scr_arrays = 0;               // plain path: per-pixel arrays the scratch offsets are laid out for
-    // list solves (dic_batch_solve_list), buffers grown on demand to list_cap subsets
-    char* d_list_io = nullptr;                // device: list, its dispatch order, initial warps
-    dic_subset_result* d_list_res = nullptr;  // device: gathered results
-    char* h_list_io = nullptr;                // pinned host: the same layout, then the results
+    // list solves (dic_batch_solve_list): pinned, device-mapped staging of
+    // the list, its dispatch order, the initial warps and the results,
+    // grown on demand to list_cap subsets
+    char* h_list_io = nullptr;
+    char* d_list_io = nullptr;                // the same memory as the device sees it
     int64_t list_cap = 0;
     std::vector<uint32_t> list_mark;          // duplicate check
     uint32_t list_stamp = 0;
@@ -133,8 +134,11 @@ struct SizeClass {
 std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm = nullptr, int64_t n = 0);
 // d_perm/h_perm/count: a dispatch order over some of the batch's subsets
 // (device and host copies); the whole batch in its own order when null
+// latency: a small list; each class takes more wavefronts per subset
+// while the whole list still fits on the chip at two wavefronts per SIMD
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
-                      const int32_t* d_perm = nullptr, const int32_t* h_perm = nullptr, int64_t count = 0);
+                      const int32_t* d_perm = nullptr, const int32_t* h_perm = nullptr, int64_t count = 0,
+                      bool latency = false);
 // times the candidate schedules of every size class and keeps the fastest
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 

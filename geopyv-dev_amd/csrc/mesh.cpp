@@ -12,6 +12,7 @@
 // warp are the reference's, so the results agree with it to the subset
 // solver's own tolerance.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -350,6 +351,7 @@ struct Run {
     std::vector<int32_t> iters;
     std::priority_queue<std::pair<uint64_t, int64_t>> heap;
     int64_t n_solves = 0, n_lists = 0;
+    double list_seconds = 0.0;  // wall time inside list solves (launches, device work, synchronisation)
     // list-solve buffers
     std::vector<int32_t> ids;
     std::vector<double> p0;
@@ -374,7 +376,9 @@ struct Run {
         r.resize(n);
         dic_solve_params p = sp;
         p.tolerance = tol;
+        const auto t0 = std::chrono::steady_clock::now();
         int st = solve_list(b, f, g, &p, ids.data(), n, p0.data(), r.data());
+        list_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (st) return st;
         n_solves += n;
         ++n_lists;
@@ -839,6 +843,7 @@ int dic_mesh_solve(dic_engine* d, const dic_image* f, const dic_image* g, int64_
     out->n_node_solves = R.n_solves;
     out->n_lists = R.n_lists;
     out->n_unsolvable = n_bad;
+    out->list_seconds = R.list_seconds;
     // the quality gate (mesh.rs:754-777), then the element strains and the
     // compatibility check (mesh.rs:798-803, 1417-1433)
     std::string err;

@@ -78,7 +78,7 @@ class MeshOutput(ctypes.Structure):
         ("p", ctypes.c_void_p), ("c_zncc", ctypes.c_void_p), ("iterations", ctypes.c_void_p),
         ("norms", ctypes.c_void_p), ("quality_ok", ctypes.c_void_p), ("warps", ctypes.c_void_p),
         ("seed_node", ctypes.c_int64), ("n_node_solves", ctypes.c_int64), ("n_lists", ctypes.c_int64),
-        ("n_unsolvable", ctypes.c_int64),
+        ("n_unsolvable", ctypes.c_int64), ("list_seconds", ctypes.c_double),
     ]
 
 
@@ -474,7 +474,7 @@ def mesh_solve(engine, f_img, g_img, nodes, elements, mesh_order, tpl_index, tpl
     res = dict(p=np.zeros((n, 12)), c_zncc=np.zeros(n), iterations=np.zeros(n, dtype=np.int32), norms=np.zeros(n),
                quality_ok=np.zeros(n, dtype=np.int32), warps=np.zeros((ne, 12)))
     out = MeshOutput(res["p"].ctypes.data, res["c_zncc"].ctypes.data, res["iterations"].ctypes.data,
-                     res["norms"].ctypes.data, res["quality_ok"].ctypes.data, res["warps"].ctypes.data, -1, 0, 0, 0)
+                     res["norms"].ctypes.data, res["quality_ok"].ctypes.data, res["warps"].ctypes.data, -1, 0, 0, 0, 0.0)
     st = L.dic_mesh_solve(engine.handle, f_img.handle, g_img.handle, n, ptr(nodes), ne, int(mesh_order), ptr(el),
                           ptr(tpl_index), len(tpl_start) - 1, ptr(tpl_start), ptr(tpl_xy), ctypes.byref(prm),
                           ctypes.byref(out))
@@ -482,5 +482,5 @@ def mesh_solve(engine, f_img, g_img, nodes, elements, mesh_order, tpl_index, tpl
     res["p"] = res["p"][:, :6 * int(subset_order)]
     res["quality_ok"] = res["quality_ok"].astype(bool)
     res.update(seed_node=int(out.seed_node), n_node_solves=int(out.n_node_solves), n_lists=int(out.n_lists),
-               n_unsolvable=int(out.n_unsolvable))
+               n_unsolvable=int(out.n_unsolvable), list_seconds=float(out.list_seconds))
     return st, msg, res

@@ -232,6 +232,7 @@ typedef struct {
     int64_t n_node_solves;   /* subset solves run, retries and corrections included */
     int64_t n_lists;         /* device list solves launched */
     int64_t n_unsolvable;    /* nodes without quality_ok after corrections */
+    double list_seconds;     /* wall time spent in the list solves (the rest is host traversal) */
 } dic_mesh_output;
 /* Seed, RG or layer-RG propagation, corrections, the quality gate, element
  * strains and the compatibility check.  Returns DIC_ERR_INVALID_INPUT for
