@@ -171,7 +171,9 @@ int dic_image_create(dic_engine* d, const void* pixels, int pix_type, int mem, i
     *out = nullptr;
     CHECK_ARG(h >= 1 && w >= 1, "invalid input: image must be at least 1x1");
     CHECK_ARG(border >= 3, "invalid input: border must be >= 3");
-    CHECK_ARG(pix_type == DIC_PIX_U8 || pix_type == DIC_PIX_F64, "invalid input: pixel type");
+    CHECK_ARG(pixel_bytes(pix_type) > 0, "invalid input: pixel type");
+    // reflect-101 reflects once (image.rs:519-528): the 5-tap blur needs 3 pixels
+    CHECK_ARG(!loader_pixels(pix_type) || (h >= 3 && w >= 3), "invalid input: a loaded image must be at least 3x3");
     CHECK_ARG(mem == DIC_MEM_HOST || mem == DIC_MEM_DEVICE, "invalid input: memory kind");
     Engine* e = &d->e;
     DIC_HIP_TRY(hipSetDevice(e->device));
@@ -203,6 +205,7 @@ int dic_image_create(dic_engine* d, const void* pixels, int pix_type, int mem, i
     if (st) {
         dev_free(e, im->coef64, np * sizeof(double));
         dev_free(e, im->coef32, np32 * sizeof(float));
+        dev_free(e, im->grey_dev, (size_t)h * w);
         delete di;
         return st;
     }
@@ -212,15 +215,30 @@ int dic_image_create(dic_engine* d, const void* pixels, int pix_type, int mem, i
 
 int dic_image_update(dic_image* di, const void* pixels, int pix_type, int mem) {
     CHECK_ARG(di && pixels, "dic_image_update: NULL argument");
-    CHECK_ARG(pix_type == DIC_PIX_U8 || pix_type == DIC_PIX_F64, "invalid input: pixel type");
+    CHECK_ARG(pixel_bytes(pix_type) > 0, "invalid input: pixel type");
     CHECK_ARG(mem == DIC_MEM_HOST || mem == DIC_MEM_DEVICE, "invalid input: memory kind");
     Image* im = &di->im;
+    CHECK_ARG(!loader_pixels(pix_type) || (im->h >= 3 && im->w >= 3), "invalid input: a loaded image must be at least 3x3");
     Engine* e = im->eng;
     DIC_HIP_TRY(hipSetDevice(e->device));
     std::lock_guard<std::mutex> lk(e->mu);
     int st = prefilter_run(e, im, pixels, pix_type, mem);
     if (st) return st;
     if (mem == DIC_MEM_HOST) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    return DIC_OK;
+}
+
+int dic_image_grey(const dic_image* di, double* out) {
+    CHECK_ARG(di && out, "dic_image_grey: NULL argument");
+    const Image* im = &di->im;
+    CHECK_ARG(im->grey_dev, "invalid input: the image was not loaded from file pixels");
+    Engine* e = im->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    std::vector<uint8_t> g((size_t)im->h * im->w);
+    DIC_HIP_TRY(hipMemcpyAsync(g.data(), im->grey_dev, g.size(), hipMemcpyDeviceToHost, e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < g.size(); ++i) out[i] = (double)g[i];
     return DIC_OK;
 }
 
@@ -252,6 +270,7 @@ void dic_image_destroy(dic_image* di) {
     dev_free(e, im->coef64, np * sizeof(double));
     dev_free(e, im->coef32, (size_t)im->hp * im->wp32 * sizeof(float));
     dev_free(e, im->pix_dev, im->pix_dev_bytes);
+    dev_free(e, im->grey_dev, (size_t)im->h * im->w);
     delete di;
 }
 
@@ -683,13 +702,14 @@ void dic_batch_destroy(dic_batch* db) {
 
 int dic_image_stage(dic_image* di, const void* pixels, int pix_type, int mem) {
     CHECK_ARG(di && pixels, "dic_image_stage: NULL argument");
-    CHECK_ARG(pix_type == DIC_PIX_U8 || pix_type == DIC_PIX_F64, "invalid input: pixel type");
+    CHECK_ARG(pixel_bytes(pix_type) > 0, "invalid input: pixel type");
     CHECK_ARG(mem == DIC_MEM_HOST || mem == DIC_MEM_DEVICE, "invalid input: memory kind");
     Image* im = &di->im;
+    CHECK_ARG(!loader_pixels(pix_type) || (im->h >= 3 && im->w >= 3), "invalid input: a loaded image must be at least 3x3");
     Engine* e = im->eng;
     DIC_HIP_TRY(hipSetDevice(e->device));
     std::lock_guard<std::mutex> lk(e->mu);
-    const size_t bytes = (size_t)im->h * im->w * (pix_type == DIC_PIX_U8 ? 1 : 8);
+    const size_t bytes = (size_t)im->h * im->w * pixel_bytes(pix_type);
     // a frame graph may hold this buffer: its type (and so its size) is
     // fixed by the first staging
     CHECK_ARG(!im->pix_dev || im->pix_dev_type == pix_type,

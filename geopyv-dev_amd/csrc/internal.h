@@ -58,7 +58,20 @@ struct Image {
     void* pix_dev = nullptr;  // staged frame pixels (dic_image_stage), h x w of pix_type
     size_t pix_dev_bytes = 0;
     int pix_dev_type = -1;
+    uint8_t* grey_dev = nullptr;  // the file loader's converted frame (h x w), loader pixel types only
 };
+
+// Bytes per pixel of a pixel type; 0 for an unknown type.
+inline size_t pixel_bytes(int pix_type) {
+    switch (pix_type) {
+        case DIC_PIX_U8: return 1;
+        case DIC_PIX_F64: return 8;
+        case DIC_PIX_RGB8_LOAD: return 3;
+        case DIC_PIX_GREY8_LOAD: return 1;
+        default: return 0;
+    }
+}
+inline bool loader_pixels(int pix_type) { return pix_type == DIC_PIX_RGB8_LOAD || pix_type == DIC_PIX_GREY8_LOAD; }
 
 struct Batch {
     Engine* eng = nullptr;

@@ -212,13 +212,70 @@ int get_twiddles(Engine* e, int n, const double2** out) {
 
 }  // namespace
 
+namespace {
+
+// The file loader's 5 Gaussian taps, sigma 1.1 (image.rs:483-491): exp of
+// -x²/(2σ²) at x = -2..2, normalised by their sum taken in order.  Host
+// libm, as the reference's f64::exp.
+struct BlurTaps {
+    double k[5];
+};
+
+BlurTaps loader_taps() {
+    const double sigma = 1.1;
+    BlurTaps t;
+    double raw[5], sum = 0.0;
+    for (int i = 0; i < 5; ++i) {
+        const double x = (double)i - 2.0;
+        raw[i] = std::exp(-x * x / (2.0 * sigma * sigma));
+    }
+    for (int i = 0; i < 5; ++i) sum += raw[i];
+    for (int i = 0; i < 5; ++i) t.k[i] = raw[i] / sum;
+    return t;
+}
+
+// The file loader's conversion of a decoded frame (image.rs:450-527): each
+// pixel's BT.709 integer grey (2126 R + 7152 G + 722 B + 5000) / 10000
+// (a grey frame is its own grey: the formula returns the value of an equal
+// R, G, B), the horizontal then the vertical 5-tap pass with reflect-101
+// borders in FP64, each sum in tap order without contraction, rounded half
+// away from zero and clamped to 0..255.  One thread per output pixel,
+// recomputing the five horizontal sums of its column window (the 25 grey
+// samples come from cache); h, w >= 3 so one reflection stays in range.
+template <int C>
+__global__ void __launch_bounds__(256) loader_convert_kernel(const uint8_t* __restrict__ src, int64_t h, int64_t w,
+                                                             BlurTaps t, uint8_t* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j >= w) return;
+    auto reflect = [](int64_t x, int64_t n) { return x < 0 ? -x : (x >= n ? 2 * (n - 1) - x : x); };
+    int64_t cols[5];
+    for (int kj = 0; kj < 5; ++kj) cols[kj] = reflect(j + kj - 2, w);
+    double acc = 0.0;
+    for (int ki = 0; ki < 5; ++ki) {
+        const uint8_t* row = src + reflect(i + ki - 2, h) * w * C;
+        double hs = 0.0;
+        for (int kj = 0; kj < 5; ++kj) {
+            const uint8_t* p = row + cols[kj] * C;
+            const uint32_t g = C == 1 ? (uint32_t)p[0]
+                                      : (2126u * p[0] + 7152u * p[1] + 722u * p[2] + 5000u) / 10000u;
+            hs = hs + t.k[kj] * (double)g;
+        }
+        acc = acc + t.k[ki] * hs;
+    }
+    out[i * w + j] = (uint8_t)fmin(fmax(round(acc), 0.0), 255.0);
+}
+
+}  // namespace
+
 int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own) {
     const int64_t h = img->h, w = img->w, hp = img->hp, wp = img->wp;
     if (wp > (int64_t)kMaxOut * 1024 || hp > (int64_t)kMaxOut * 1024) {
         set_error("prefilter: padded line longer than 8192 samples");
         return DIC_ERR_INVALID_INPUT;
     }
-    const size_t pix_bytes = (size_t)h * w * (pix_type == DIC_PIX_U8 ? 1 : 8);
+    const size_t pix_bytes = (size_t)h * w * pixel_bytes(pix_type);
     const void* dpix = pix;
     if (mem == DIC_MEM_HOST) {
         if (e->pix_stage_bytes < pix_bytes) {
@@ -252,7 +309,23 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
     const int tr = threads_for((int)wp), tc = threads_for((int)hp);
     const size_t lr = sizeof(double2) * wp, lc = sizeof(double2) * hp;
     double* tmp = tmp_own ? tmp_own : e->pre_tmp;
+    if (loader_pixels(pix_type) && !img->grey_dev) {
+        const int st0 = dev_alloc(e, (void**)&img->grey_dev, (size_t)h * w);
+        if (st0) return st0;
+    }
     if (!e->capturing) DIC_HIP_TRY(hipEventRecord(e->ev_pre0, e->stream));
+    if (loader_pixels(pix_type)) {
+        // decoded file pixels: the loader's conversion, then the prefilter of
+        // its 8-bit result
+        const dim3 grid((unsigned)((w + 255) / 256), (unsigned)h);
+        if (pix_type == DIC_PIX_RGB8_LOAD)
+            loader_convert_kernel<3><<<grid, 256, 0, e->stream>>>((const uint8_t*)dpix, h, w, loader_taps(), img->grey_dev);
+        else
+            loader_convert_kernel<1><<<grid, 256, 0, e->stream>>>((const uint8_t*)dpix, h, w, loader_taps(), img->grey_dev);
+        DIC_HIP_TRY(hipGetLastError());
+        dpix = img->grey_dev;
+        pix_type = DIC_PIX_U8;
+    }
     const int row_blocks = (int)((hp + 1) / 2);
     if (pix_type == DIC_PIX_U8) {
         DIC_HIP_TRY(hipFuncSetAttribute((const void*)prefilter_rows_kernel<uint8_t>,

@@ -16,7 +16,7 @@ import numpy as np
 from . import _native
 from ._native import GeodicError
 
-__all__ = ["Image", "Mask", "Subset", "Mesh", "structured_mesh", "quadratic_mesh", "solve_subsets", "GeodicError"]
+__all__ = ["Image", "Mask", "Subset", "Mesh", "Sequence", "structured_mesh", "quadratic_mesh", "solve_subsets", "GeodicError"]
 
 
 def _engine():
@@ -26,41 +26,14 @@ def _engine():
 # ---------------------------------------------------------------------------
 # Image (python/src/py_image.rs:34-122)
 # ---------------------------------------------------------------------------
-def load_grayscale(path):
-    """Image::from_file's loader (image.rs:450-527): BT.709 integer grey
-    (2126 R + 7152 G + 722 B + 5000) / 10000, then a 5x5 Gaussian blur with
-    σ = 1.1 and reflect-101 borders, rounded and clamped to 0-255."""
+def decode_frame(path):
+    """Image::from_file's decode step (image.rs:450-456): the file's pixels
+    as 8-bit RGB, (h, w, 3) uint8.  The grey conversion and the blur run on
+    the device (DIC_PIX_RGB8_LOAD)."""
     from PIL import Image as _PIL
 
     with _PIL.open(path) as im:
-        rgb = np.asarray(im.convert("RGB"), dtype=np.uint32)
-    gs = ((2126 * rgb[..., 0] + 7152 * rgb[..., 1] + 722 * rgb[..., 2] + 5000) // 10000).astype(np.float64)
-    return gaussian_blur_5x5(gs)
-
-
-def gaussian_blur_5x5(gs, sigma=1.1):
-    """image.rs:477-527: separable 5-tap Gaussian, reflect-101, round, clamp."""
-    x = np.arange(5) - 2.0
-    raw = np.exp(-x * x / (2.0 * sigma * sigma))
-    s = 0.0
-    for v in raw:
-        s += v
-    k = raw / s
-    rows, cols = gs.shape
-
-    def reflect(i, n):
-        i = np.where(i < 0, -i, i)
-        return np.where(i >= n, 2 * (n - 1) - i, i)
-
-    hpass = np.zeros_like(gs)
-    jj = np.arange(cols)
-    for ki in range(5):
-        hpass = hpass + k[ki] * gs[:, reflect(jj + ki - 2, cols)]
-    out = np.zeros_like(gs)
-    ii = np.arange(rows)
-    for ki in range(5):
-        out = out + k[ki] * hpass[reflect(ii + ki - 2, rows), :]
-    return np.clip(np.round(out), 0.0, 255.0)
+        return np.ascontiguousarray(np.asarray(im.convert("RGB"), dtype=np.uint8))
 
 
 class Image:
@@ -74,26 +47,33 @@ class Image:
             raise ValueError("one of `filepath` or `image_gs` is required")
         if border < 0:
             raise OverflowError("can't convert negative int to unsigned")
+        self._border = int(border)
         if filepath is not None:
             fp = os.fspath(filepath)
             if not os.path.exists(fp):
                 raise RuntimeError(f"file not found: {fp}")
             try:
-                gs = load_grayscale(fp)
+                rgb = decode_frame(fp)
             except Exception as exc:  # decoding failure -> Error::ImageLoad
                 raise RuntimeError(f"image load error: {exc}") from None
             self._filepath = fp
+            # grey conversion, blur and prefilter on the device (image.rs:69-77)
+            self._dev = _native.DeviceImage(_engine(), pixels=rgb, border=self._border, load=True)
+            self._gs = None
+            self._shape = rgb.shape[:2]
         else:
             gs = np.array(image_gs, dtype=np.float64, copy=True)
             if gs.ndim != 2:
                 raise TypeError("image_gs must be a 2-D float64 array")
             self._filepath = None
-        self._gs = gs
-        self._border = int(border)
-        self._dev = _native.DeviceImage(_engine(), pixels=gs, border=self._border)
+            self._gs = gs
+            self._shape = gs.shape
+            self._dev = _native.DeviceImage(_engine(), pixels=gs, border=self._border)
 
     @property
     def image_gs(self):
+        if self._gs is None:  # the loader's output, read back once
+            self._gs = self._dev.grey()
         return self._gs.copy()
 
     @property
@@ -116,14 +96,14 @@ class Image:
         from ._bspline import Q
 
         C = self.coef
-        h, w = self._gs.shape
+        h, w = self._shape
         b = self._border
         win = np.lib.stride_tricks.sliding_window_view(C, (6, 6))[b - 2:b - 2 + h, b - 2:b - 2 + w]
         blocks = np.einsum("rk,ijkl,cl->ijrc", Q, win, Q)
         return blocks.transpose(0, 2, 1, 3).reshape(h * 6, w * 6)
 
     def __repr__(self):
-        h, w = self._gs.shape
+        h, w = self._shape
         return f"Image(shape=({h}, {w}), border={self._border})"
 
 
@@ -583,3 +563,4 @@ def solve_sequence(frames, coords, local_mask, subset_order=1, reference_every=2
 
 
 from .mesh import Mesh, quadratic_mesh, structured_mesh  # noqa: E402  (uses Image, Mask and the solver helpers above)
+from .sequence import Sequence  # noqa: E402

@@ -27,7 +27,7 @@ ERR_SINGULAR_WARP = 11
 ERR_DEVICE = 20
 ERR_OUT_OF_MEMORY = 21
 
-PIX_U8, PIX_F64 = 0, 1
+PIX_U8, PIX_F64, PIX_RGB8_LOAD, PIX_GREY8_LOAD = 0, 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
 SHAPE = {"circle": 0, "square": 1, "semicircle": 2}
 METHOD = {"icgn": 0, "fagn": 1}
@@ -43,6 +43,7 @@ EXPORTED = [
     "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
     "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
     "dic_frame_graph_destroy", "dic_batch_solve_list", "dic_mesh_solve", "dic_mesh_element_strains",
+    "dic_image_grey",
 ]
 
 
@@ -129,6 +130,7 @@ def load():
                                        ctypes.c_int, ctypes.POINTER(_vp)]
         L.dic_image_update.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
         L.dic_image_coef.argtypes = [_vp, _vp]
+        L.dic_image_grey.argtypes = [_vp, _vp]
         L.dic_image_info.argtypes = [_vp, _i64p, _i64p, ctypes.POINTER(ctypes.c_int)]
         L.dic_image_destroy.argtypes = [_vp]
         L.dic_image_destroy.restype = None
@@ -162,6 +164,10 @@ def load():
         L.dic_mesh_element_strains.argtypes = [ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp]
         _lib = L
         return L
+
+
+def load_lib():
+    return load()
 
 
 def check(status):
@@ -239,11 +245,21 @@ class Engine:
         return load().dic_engine_bytes_in_use(self.handle)
 
 
+def _pix_type(pixels, load):
+    """Pixel type of a host array: decoded file pixels when `load` (the
+    loader's grey conversion and blur run on the device), else grey."""
+    if load:
+        if pixels.dtype != np.uint8 or pixels.ndim not in (2, 3) or (pixels.ndim == 3 and pixels.shape[2] != 3):
+            raise ValueError("decoded file pixels must be (h, w, 3) or (h, w) uint8")
+        return PIX_RGB8_LOAD if pixels.ndim == 3 else PIX_GREY8_LOAD
+    return PIX_U8 if pixels.dtype == np.uint8 else PIX_F64
+
+
 class DeviceImage:
     """Device-resident B-spline coefficients of one image."""
 
-    def __init__(self, engine, pixels=None, device_ptr=None, shape=None, pix_type=None, border=20):
-        L = load()
+    def __init__(self, engine, pixels=None, device_ptr=None, shape=None, pix_type=None, border=20, load=False):
+        L = load_lib()
         self.engine = engine
         h = _vp()
         if device_ptr is not None:
@@ -252,41 +268,46 @@ class DeviceImage:
                                      int(border), ctypes.byref(h)))
         else:
             pixels = np.ascontiguousarray(pixels)
-            if pixels.dtype == np.uint8:
-                pt = PIX_U8
-            else:
+            pt = _pix_type(pixels, load)
+            if pt == PIX_F64:
                 pixels = np.ascontiguousarray(pixels, dtype=np.float64)
-                pt = PIX_F64
-            hh, ww = pixels.shape
+            hh, ww = pixels.shape[:2]
             check(L.dic_image_create(engine.handle, ptr(pixels), pt, MEM_HOST, hh, ww, int(border),
                                      ctypes.byref(h)))
         self.handle = h
         self.shape = (int(hh), int(ww))
         self.border = int(border)
 
-    def update(self, pixels=None, device_ptr=None, pix_type=None):
-        L = load()
+    def update(self, pixels=None, device_ptr=None, pix_type=None, load=False):
+        L = load_lib()
         if device_ptr is not None:
             check(L.dic_image_update(self.handle, ctypes.c_void_p(device_ptr), pix_type, MEM_DEVICE))
         else:
             pixels = np.ascontiguousarray(pixels)
-            pt = PIX_U8 if pixels.dtype == np.uint8 else PIX_F64
+            pt = _pix_type(pixels, load)
             if pt == PIX_F64:
                 pixels = np.ascontiguousarray(pixels, dtype=np.float64)
             check(L.dic_image_update(self.handle, ptr(pixels), pt, MEM_HOST))
 
-    def stage(self, pixels=None, device_ptr=None, pix_type=None):
+    def stage(self, pixels=None, device_ptr=None, pix_type=None, load=False):
         """Copy a frame's pixels into the image's device pixel buffer (the
         source its frame graph prefilters)."""
-        L = load()
+        L = load_lib()
         if device_ptr is not None:
             check(L.dic_image_stage(self.handle, ctypes.c_void_p(device_ptr), pix_type, MEM_DEVICE))
         else:
             pixels = np.ascontiguousarray(pixels)
-            pt = PIX_U8 if pixels.dtype == np.uint8 else PIX_F64
+            pt = _pix_type(pixels, load)
             if pt == PIX_F64:
                 pixels = np.ascontiguousarray(pixels, dtype=np.float64)
             check(L.dic_image_stage(self.handle, ptr(pixels), pt, MEM_HOST))
+
+    def grey(self):
+        """The grey frame an image loaded from file pixels was prefiltered
+        from (h, w) float64."""
+        out = np.empty(self.shape, dtype=np.float64)
+        check(load_lib().dic_image_grey(self.handle, ptr(out)))
+        return out
 
     def coef(self):
         h, w = self.shape

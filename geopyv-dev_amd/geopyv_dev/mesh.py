@@ -154,9 +154,18 @@ class Mesh:
             raise ValueError(f"elements must be (M, {3 if mesh_order == 1 else 6}) for mesh_order {mesh_order}")
         self._mesh_order = int(mesh_order)
         self._f_img, self._g_img = f_img, g_img
-        self._mask = roi_mask(f_img.image_gs.shape, boundary, boundary_hard, exclusions, exclusions_hard)
+        self._mask = roi_mask(f_img._shape, boundary, boundary_hard, exclusions, exclusions_hard)
         self._time = time
         self._soil = soil_properties
+        self._solution = None
+        self._templates = None  # (local mask key, tpl_index, tpl_start, tpl_xy)
+
+    def _use_images(self, f_img, g_img):
+        """Another image pair of the same shape on the same geometry (a
+        sequence's next pair): the per-node pixel lists are kept."""
+        if f_img._shape != self._f_img._shape or g_img._shape != self._f_img._shape:
+            raise ValueError("images of another shape need another mesh")
+        self._f_img, self._g_img = f_img, g_img
         self._solution = None
 
     # ---- geometry ----
@@ -236,7 +245,10 @@ class Mesh:
             raise ValueError("a mesh solves subset_order 1 or 2")
         if precision not in ("f64", "f32"):
             raise ValueError("precision must be 'f64' or 'f32'")
-        tpl_index, tpl_start, tpl_xy = node_templates(local_mask, self._nodes, self._mask)
+        key = (local_mask.shape, local_mask.size, local_mask.coords.tobytes())
+        if self._templates is None or self._templates[0] != key:
+            self._templates = (key,) + node_templates(local_mask, self._nodes, self._mask)
+        _, tpl_index, tpl_start, tpl_xy = self._templates
         m = _P_LEN[subset_order]
         warp = np.zeros(m) if seed_warp is None else np.asarray(seed_warp, dtype=np.float64).ravel()
         eng = _native.Engine.get(0)

@@ -40,8 +40,15 @@ typedef enum {
     DIC_ERR_BUSY = 22                /* serving queue full */
 } dic_status;
 
-/* Pixel element types for image upload. */
-enum { DIC_PIX_U8 = 0, DIC_PIX_F64 = 1 };
+/* Pixel element types for image upload.  U8 and F64 are grey images taken
+ * as they are (Image::from_array, image.rs:83-86).  RGB8_LOAD (h, w, 3
+ * interleaved) and GREY8_LOAD (h, w) are decoded file pixels: the file
+ * loader's conversion (load_grayscale + gaussian_blur_5x5, image.rs:450-527:
+ * BT.709 integer grey (2126 R + 7152 G + 722 B + 5000) / 10000, a 5x5
+ * Gaussian with sigma 1.1 and reflect-101 borders in FP64, rounded and
+ * clamped to 0-255) runs on the device before the prefilter, as
+ * Image::from_file (image.rs:69-77) does; these need h, w >= 3. */
+enum { DIC_PIX_U8 = 0, DIC_PIX_F64 = 1, DIC_PIX_RGB8_LOAD = 2, DIC_PIX_GREY8_LOAD = 3 };
 /* Where a buffer lives. */
 enum { DIC_MEM_HOST = 0, DIC_MEM_DEVICE = 1 };
 /* Local mask shapes: masks.rs:21-31 MaskShape. */
@@ -109,6 +116,10 @@ int dic_image_update(dic_image* img, const void* pixels, int pix_type, int mem);
  * return.  Per-frame ingest of sequence.rs:693-694,914 (the decode stays on
  * the host). */
 int dic_image_stage(dic_image* img, const void* pixels, int pix_type, int mem);
+/* The grey pixels an image loaded from file pixels (DIC_PIX_*_LOAD) was
+ * prefiltered from, h*w doubles (Image.image_gs, python/src/py_image.rs);
+ * DIC_ERR_INVALID_INPUT for an image given as grey pixels. */
+int dic_image_grey(const dic_image* img, double* out);
 /* Copies C back to the host: (h+2b)*(w+2b) doubles (Image.coef, image.rs:51). */
 int dic_image_coef(const dic_image* img, double* out);
 int dic_image_info(const dic_image* img, int64_t* h, int64_t* w, int* border);

@@ -151,6 +151,43 @@ static void build_kernel(int n, std::vector<cplx>& k) {
     k[n - 1] = cplx(kk[1], 0.0);
 }
 
+// load_grayscale + gaussian_blur_5x5 + reflect101 (image.rs:450-527): the
+// file loader's BT.709 integer grey of a decoded RGB8 frame (channels 3) or
+// a grey frame (channels 1: to_rgb8 replicates it, and the formula returns
+// the value), then the horizontal and vertical 5-tap Gaussian passes
+// (sigma 1.1, taps exp(-x²/(2σ²)) normalised by their in-order sum), each
+// sum in tap order, rounded half away from zero and clamped to 0..255.
+void orc_load_convert(const uint8_t* src, int h, int w, int channels, double* out) {
+    const double sigma = 1.1;
+    double raw[5], k[5], sum = 0.0;
+    for (int i = 0; i < 5; ++i) {
+        const double x = (double)i - 2.0;
+        raw[i] = std::exp(-x * x / (2.0 * sigma * sigma));
+    }
+    for (int i = 0; i < 5; ++i) sum += raw[i];
+    for (int i = 0; i < 5; ++i) k[i] = raw[i] / sum;
+    auto reflect101 = [](long idx, long len) { return idx < 0 ? -idx : (idx >= len ? 2 * (len - 1) - idx : idx); };
+    std::vector<double> gs((size_t)h * w), hp((size_t)h * w);
+    for (long y = 0; y < h; ++y)
+        for (long x = 0; x < w; ++x) {
+            const uint8_t* p = src + (y * w + x) * channels;
+            const uint32_t r = p[0], g = channels == 3 ? p[1] : p[0], b = channels == 3 ? p[2] : p[0];
+            gs[y * w + x] = (double)((2126 * r + 7152 * g + 722 * b + 5000) / 10000);
+        }
+    for (long i = 0; i < h; ++i)
+        for (long j = 0; j < w; ++j) {
+            double v = 0.0;
+            for (long ki = 0; ki < 5; ++ki) v += k[ki] * gs[i * w + reflect101(j + ki - 2, w)];
+            hp[i * w + j] = v;
+        }
+    for (long i = 0; i < h; ++i)
+        for (long j = 0; j < w; ++j) {
+            double v = 0.0;
+            for (long ki = 0; ki < 5; ++ki) v += k[ki] * hp[reflect101(i + ki - 2, h) * w + j];
+            out[i * w + j] = std::min(std::max(std::round(v), 0.0), 255.0);
+        }
+}
+
 // pad_replicate (image.rs:436-443)
 void orc_pad_replicate(const double* img, int rows, int cols, int border, double* out) {
     int pr = rows + 2 * border, pc = cols + 2 * border;

@@ -410,9 +410,123 @@ int element_strains(const double* nodes, const Mesh& m, const double* disp, doub
     return ST_OK;
 }
 
+// particle.rs:137-183 local_coordinates: area coordinates (zeta, eta,
+// theta) of a point in the corner triangle, and the denominator determinant.
+double det3(double a00, double a01, double a02, double a10, double a11, double a12, double a20, double a21,
+            double a22) {
+    return a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20) + a02 * (a10 * a21 - a11 * a20);
+}
+
+void local_coords(const double* c, const double* en, double* zeta, double* eta, double* theta, double* den) {
+    const double px = c[0], py = c[1];
+    const double n0x = en[0], n0y = en[1], n1x = en[2], n1y = en[3], n2x = en[4], n2y = en[5];
+    *den = det3(1.0, 1.0, 1.0, n0x, n1x, n2x, n0y, n1y, n2y);
+    const double dz = det3(1.0, 1.0, 1.0, px, n1x, n2x, py, n1y, n2y);
+    const double de = det3(1.0, 1.0, 1.0, px, n2x, n0x, py, n2y, n0y);
+    *zeta = dz / *den;
+    *eta = de / *den;
+    *theta = 1.0 - *zeta - *eta;
+}
+
 }  // namespace
 
 extern "C" {
+
+// warp_increment (particle.rs:254-320) with shape_function (186-238): the
+// displacement and its gradients (and for 6-node elements the second
+// gradients) at a point, interpolated in one element.
+void orc_warp_increment(const double* coord, const double* en, const double* ed, int order, double* out) {
+    double z, e, t, den;
+    local_coords(coord, en, &z, &e, &t, &den);
+    const int ns = order == 1 ? 3 : 6;
+    double N[6], dN[2][6];
+    if (order == 1) {
+        N[0] = z; N[1] = e; N[2] = t;
+        const double d[2][3] = {{1.0, 0.0, -1.0}, {0.0, 1.0, -1.0}};
+        for (int r = 0; r < 2; ++r)
+            for (int k = 0; k < 3; ++k) dN[r][k] = d[r][k];
+    } else {
+        N[0] = z * (2.0 * z - 1.0);
+        N[1] = e * (2.0 * e - 1.0);
+        N[2] = t * (2.0 * t - 1.0);
+        N[3] = 4.0 * z * e;
+        N[4] = 4.0 * e * t;
+        N[5] = 4.0 * t * z;
+        const double d[2][6] = {{4.0 * z - 1.0, 0.0, 1.0 - 4.0 * t, 4.0 * e, -4.0 * e, 4.0 * (t - z)},
+                                {0.0, 4.0 * e - 1.0, 1.0 - 4.0 * t, 4.0 * z, 4.0 * (t - e), -4.0 * z}};
+        for (int r = 0; r < 2; ++r)
+            for (int k = 0; k < 6; ++k) dN[r][k] = d[r][k];
+    }
+    const int plen = 6 * order;
+    for (int k = 0; k < plen; ++k) out[k] = 0.0;
+    for (int c = 0; c < 2; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < ns; ++k) s += N[k] * ed[2 * k + c];
+        out[c] = s;
+    }
+    double jx[2][2], ju[2][2];
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) {
+            double a = 0.0, b = 0.0;
+            for (int k = 0; k < ns; ++k) a += dN[r][k] * en[2 * k + c];
+            for (int k = 0; k < ns; ++k) b += dN[r][k] * ed[2 * k + c];
+            jx[r][c] = a;
+            ju[r][c] = b;
+        }
+    // inv2x2 (particle.rs:323-333): reciprocal of the determinant
+    const double idet = 1.0 / (jx[0][0] * jx[1][1] - jx[0][1] * jx[1][0]);
+    const double ij[2][2] = {{jx[1][1] * idet, -jx[0][1] * idet}, {-jx[1][0] * idet, jx[0][0] * idet}};
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) out[2 + 2 * r + c] = ij[r][0] * ju[0][c] + ij[r][1] * ju[1][c];
+    if (order == 2) {
+        static const double d2N[3][6] = {
+            {4.0, 0.0, 4.0, 0.0, 0.0, -8.0}, {0.0, 0.0, 4.0, 4.0, -4.0, -4.0}, {0.0, 4.0, 4.0, 0.0, -8.0, 0.0}};
+        double ku[3][2];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 2; ++c) {
+                double a = 0.0;
+                for (int k = 0; k < 6; ++k) a += d2N[r][k] * ed[2 * k + c];
+                ku[r][c] = a;
+            }
+        const double dz[2][2] = {{(en[3] - en[5]) / den, (en[5] - en[1]) / den},
+                                 {(en[4] - en[2]) / den, (en[0] - en[4]) / den}};
+        const double kx[3][3] = {{dz[0][0] * dz[0][0], 2.0 * dz[0][0] * dz[0][1], dz[0][1] * dz[0][1]},
+                                 {dz[0][0] * dz[1][0], dz[0][0] * dz[1][1] + dz[0][1] * dz[1][0], dz[0][1] * dz[1][1]},
+                                 {dz[1][0] * dz[1][0], 2.0 * dz[1][0] * dz[1][1], dz[1][1] * dz[1][1]}};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 2; ++c) out[6 + 2 * r + c] = kx[r][0] * ku[0][c] + kx[r][1] * ku[1][c] + kx[r][2] * ku[2][c];
+    }
+}
+
+// element_locator (particle.rs:968-1013): elements by squared centroid
+// distance (stable order), the first whose corner triangle holds the point
+// (area coordinates all >= 0); else the nearest, flagged adrift.
+int64_t orc_element_locator(const double* coord, const double* nodes, const int64_t* el, int64_t n_el, int cols,
+                            const double* centroids, int* adrift) {
+    std::vector<std::pair<double, int64_t>> d(n_el);
+    for (int64_t i = 0; i < n_el; ++i) {
+        const double dx = centroids[2 * i] - coord[0], dy = centroids[2 * i + 1] - coord[1];
+        d[i] = {dx * dx + dy * dy, i};
+    }
+    std::stable_sort(d.begin(), d.end(), [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+        return a.first < b.first;
+    });
+    for (auto& q : d) {
+        double en[6];
+        for (int k = 0; k < 3; ++k) {
+            en[2 * k] = nodes[2 * el[q.second * cols + k]];
+            en[2 * k + 1] = nodes[2 * el[q.second * cols + k] + 1];
+        }
+        double z, e, t, den;
+        local_coords(coord, en, &z, &e, &t, &den);
+        if (z >= 0.0 && e >= 0.0 && t >= 0.0) {
+            *adrift = 0;
+            return q.second;
+        }
+    }
+    *adrift = 1;
+    return d.empty() ? 0 : d[0].second;
+}
 
 // Parameters of one mesh solve: SolveConfig (mesh.rs:222-248), SeedConfig
 // (269-278) and LayerRgConfig (339-356).  preconditioning 0 = RG, 1 = layer-RG.

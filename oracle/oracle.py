@@ -12,6 +12,11 @@ Rust crate, which cannot be built here):
 function               pinned by
 =====================  ==================================================
 fft                    direct O(n^2) DFT and numpy.fft on every config length
+load_convert           BT.709 integer grey known answers; constant frame
+                       kept; a single bright pixel gives the rounded
+                       outer product of the taps; reflect-101 borders and
+                       the interior against an independent numpy reading
+                       of image.rs:450-527
 get_c                  constant image -> constant C (image.rs:538-550);
                        circular re-convolution of C with the 5-tap kernel
                        reproduces the padded image; dense circulant solve
@@ -136,6 +141,17 @@ def fft(x, sign=-1):
 def unrolled_sum(x):
     x = _c64(x)
     return lib().orc_unrolled_sum(_d(x), len(x))
+
+
+def load_convert(pixels):
+    """Image::from_file's conversion of decoded pixels (image.rs:450-527):
+    (h, w, 3) RGB8 or (h, w) grey8 -> blurred grey, float64 0..255."""
+    src = np.ascontiguousarray(pixels, dtype=np.uint8)
+    h, w = src.shape[:2]
+    ch = 3 if src.ndim == 3 else 1
+    out = np.zeros((h, w))
+    lib().orc_load_convert(src.ctypes.data_as(_u8p), h, w, ch, _d(out))
+    return out
 
 
 def pad_replicate(img, border):
@@ -519,3 +535,24 @@ def mesh_solve(fcoef, gcoef, shape, border, nodes, elements, tpl_index, tpl_xy, 
     res["quality_ok"] = res["quality_ok"].astype(bool)
     res.update(seed_node=int(out.seed_node), n_solves=int(out.n_solves), n_bad=int(out.n_bad))
     return st, res
+
+
+def warp_increment(coord, element_nodes, element_disps, mesh_order):
+    """particle.rs:254-320: warp of the displacement field at a point of
+    one element (6 * mesh_order values)."""
+    en, ed = _c64(element_nodes), _c64(element_disps)
+    out = np.zeros(6 * mesh_order)
+    lib().orc_warp_increment(_d(_c64(coord)), _d(en), _d(ed), mesh_order, _d(out))
+    return out
+
+
+def element_locator(coord, nodes, elements, centroids):
+    """particle.rs:968-990: (element index, adrift)."""
+    el, ne, cols, _ = _el(elements)
+    nodes, cen = _c64(nodes), _c64(centroids)
+    adrift = ctypes.c_int()
+    L = lib()
+    L.orc_element_locator.restype = ctypes.c_int64
+    idx = L.orc_element_locator(_d(_c64(coord)), _d(nodes), el.ctypes.data_as(_i64p), ctypes.c_int64(ne), cols,
+                                _d(cen), ctypes.byref(adrift))
+    return int(idx), bool(adrift.value)

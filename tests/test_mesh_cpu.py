@@ -321,3 +321,55 @@ def test_layer_rg_is_deterministic(band_case):
         np.testing.assert_array_equal(r["p"], runs[0]["p"])
         np.testing.assert_array_equal(r["c_zncc"], runs[0]["c_zncc"])
         np.testing.assert_array_equal(r["iterations"], runs[0]["iterations"])
+
+
+# ---------------------------------------------------------------------------
+# the sequence's guide step: warp_increment / element_locator
+# (particle.rs:254-320, 968-1013; warp_convention_tests 2648-2669)
+# ---------------------------------------------------------------------------
+def test_warp_increment_every_slot():
+    nodes = six_node_element()
+    d = np.array([quad_warp(x, y)[:2] for x, y in nodes])
+    np.testing.assert_allclose(O.warp_increment([4.0, 4.0], nodes, d, 2), quad_warp(4.0, 4.0), rtol=0, atol=1e-12)
+    lin = lambda x, y: [Q[0] + Q[2] * x + Q[4] * y, Q[1] + Q[3] * x + Q[5] * y]
+    d1 = np.array([lin(x, y) for x, y in nodes[:3]])
+    np.testing.assert_allclose(O.warp_increment([4.0, 4.0], nodes[:3], d1, 1), lin(4.0, 4.0) + Q[2:6], rtol=0,
+                               atol=1e-12)
+
+
+def test_library_guide_step_equals_oracle():
+    from geopyv_dev import sequence as S
+    rng = np.random.default_rng(11)
+    for order in (1, 2):
+        for _ in range(20):
+            nodes = six_node_element() + rng.normal(scale=0.3, size=(6, 2))
+            d = rng.normal(size=(6, 2))
+            c = nodes[:3].mean(axis=0) + rng.normal(scale=0.5, size=2)
+            k = 3 if order == 1 else 6
+            np.testing.assert_allclose(S.warp_increment(c, nodes[:k], d[:k], order),
+                                       O.warp_increment(c, nodes[:k], d[:k], order), rtol=1e-12, atol=1e-12)
+
+
+def test_element_locator():
+    from geopyv_dev import sequence as S
+    from geopyv_dev.mesh import element_centroids
+    nodes, el = band_grid(6, 0.0, 10.0)
+    cen = element_centroids(nodes, el)
+    for c in ([12.0, 13.0], [10.0, 10.0], [25.0, 20.0], [49.9, 0.1], [-5.0, 20.0], [70.0, 70.0]):
+        got = S.element_locator(np.array(c), nodes, el, cen)
+        assert got == O.element_locator(c, nodes, el, cen)
+        i, adrift = got
+        inside = c[0] >= 0 and c[1] >= 0 and c[0] <= 50 and c[1] <= 50
+        assert adrift == (not inside)
+    # a point inside: its element's area coordinates are all >= 0
+    i, adrift = S.element_locator(np.array([12.0, 13.0]), nodes, el, cen)
+    z, e, t, _ = S._local([12.0, 13.0], nodes[el[i]])
+    assert min(z, e, t) >= 0 and not adrift
+
+
+def test_image_files_in_reference_order(tmp_path):
+    from geopyv_dev.sequence import image_files
+    for name in ("frame_10.png", "frame_2.jpg", "img3_001.JPEG", "notes.txt", "frame_0.png"):
+        (tmp_path / name).write_bytes(b"")
+    got = [p.split("/")[-1] for p in image_files(str(tmp_path))]
+    assert got == ["frame_0.png", "img3_001.JPEG", "frame_2.jpg", "frame_10.png"]
