@@ -94,6 +94,15 @@ def run_shear_band(args, torch, gp, _native, workloads):
     torch.cuda.synchronize()
     total_s = time.perf_counter() - tb
     res = batch.results()
+    # one frame of the plain FP64 drop-in on the same batch, for the ratio
+    batch.set_p0(p0)
+    t = time.perf_counter()
+    batch.solve(f_img, g_img, method="icgn", precision=_native.PREC_PLAIN_F64, max_iterations=args.max_iterations,
+                tolerance=0.75, max_norm=1e-5)
+    eng.sync()
+    plain_ms = eng.timing()[1]
+    log(f"plain FP64 frame: kernel {plain_ms:.1f} ms, wall {(time.perf_counter() - t) * 1e3:.1f} ms")
+    res64 = batch.results()
     out = {
         "metric": "ICGN subsets/sec, uneven radii 15-40 px, second-order warp (FP32 engine)",
         "value": round(n / (total_s / args.steps), 1), "unit": "subsets/s", "n_gpus": 1, "steps": args.steps,
@@ -105,21 +114,33 @@ def run_shear_band(args, torch, gp, _native, workloads):
         "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                              "p99": round(float(np.percentile(lat, 99)), 4)},
         "kernel_ms": round(float(np.mean(sol)), 4),
+        "fp64_plain_kernel_ms": round(plain_ms, 4),
+        "speedup_vs_fp64_plain_kernel": round(plain_ms / float(np.mean(sol)), 3),
         "iterations_mean": round(float(np.mean(res["iterations"])), 3),
         "converged_fraction": round(float(np.mean(res["converged"])), 5),
+        "max_abs_du_f32_vs_f64_px": float(np.abs(res["p"][:, :2] - res64["p"][:, :2])[
+            (res["converged"] == 1) & (res64["converged"] == 1)].max(initial=0.0)),
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        idx = np.linspace(0, n - 1, min(1000, n)).astype(np.int64)
         threads = max(1, min(16, os.cpu_count() or 1))
         Cf = O.get_c(ref.astype(np.float64), border, threads)
         Cg = O.get_c(tar.astype(np.float64), border, threads)
-        t = O.wall()
-        orc = O.batch_solve(Cf, Cg, ref.shape, border, coords[idx], index[idx], xy.astype(np.float64), start,
-                            np.full(len(idx), 2, np.int32), p0[idx], max_norm=1e-5, max_iterations=50,
-                            threads=threads)
-        cpu_s = O.wall() - t
+
+        def cpu_sample(k):
+            idx = np.linspace(0, n - 1, min(k, n)).astype(np.int64)
+            t = O.wall()
+            orc = O.batch_solve(Cf, Cg, ref.shape, border, coords[idx], index[idx], xy.astype(np.float64), start,
+                                np.full(len(idx), 2, np.int32), p0[idx], max_norm=1e-5, max_iterations=50,
+                                threads=threads)
+            return idx, orc, O.wall() - t
+
+        # a probe of 1000 subsets sizes the sample to about --cpu-seconds
+        idx, orc, cpu_s = cpu_sample(1000)
+        k = int(len(idx) / max(cpu_s, 1e-6) * args.cpu_seconds)
+        if k > len(idx):
+            idx, orc, cpu_s = cpu_sample(k)
         d = res[idx]
         both = (d["converged"] == 1) & (orc["converged"] == 1)
         out["cpu_baseline"] = {"value": round(len(idx) / cpu_s, 2), "unit": "subsets/s", "cores": threads,
