@@ -12,12 +12,17 @@
 // warp are the reference's, so the results agree with it to the subset
 // solver's own tolerance.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <queue>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -337,6 +342,76 @@ int strains(const double* nodes, const Topology& t, const double* disp, double* 
     return DIC_OK;
 }
 
+// Host workers for the per-node preparation of large lists (the initial
+// warps of a colour read only solved nodes, so its nodes are independent).
+// Created on first use; the calling thread works too.
+class Workers {
+  public:
+    ~Workers() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    template <class F>
+    void run(int64_t n, F&& f) {
+        if (n < 256) {
+            for (int64_t i = 0; i < n; ++i) f(i);
+            return;
+        }
+        if (th_.empty()) start();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = [&f](int64_t i) { f(i); };
+            n_ = n;
+            next_ = 0;
+            busy_ = (int)th_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return busy_ == 0; });
+        job_ = nullptr;
+    }
+
+  private:
+    void start() {
+        const int hc = (int)std::thread::hardware_concurrency();  // 0 when unknown
+        const int nt = std::max(1, std::min(16, hc) - 1);
+        for (int k = 0; k < nt; ++k) th_.emplace_back([this] { loop(); });
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    void work() {
+        for (int64_t i; (i = next_.fetch_add(32)) < n_;)
+            for (int64_t j = i; j < std::min(i + 32, n_); ++j) job_(j);
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::function<void(int64_t)> job_;
+    int64_t n_ = 0;
+    std::atomic<int64_t> next_{0};
+    int busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 // The traversal's per-node state (the arrays of solve_impl, mesh.rs:639-648).
 struct Run {
     const Topology* t;
@@ -357,6 +432,7 @@ struct Run {
     std::vector<double> p0;
     std::vector<dic_subset_result> ra, rb;
     std::vector<int64_t> retry;
+    Workers workers;
 
     std::vector<int64_t> trusted(int64_t i, const uint8_t* skip) const {
         std::vector<int64_t> out;
@@ -495,9 +571,10 @@ struct Run {
             for (size_t k = 0; k < cand.size(); ++k)
                 if (colour[k] == c) members.push_back(cand[k]);
             pa.assign(members.size() * plen, 0.0);
-            for (size_t k = 0; k < members.size(); ++k)
+            workers.run((int64_t)members.size(), [&](int64_t k) {
                 initial_warp(nodes, &nodes[2 * members[k]], trusted(members[k], nullptr), czn.data(), P.data(), plen,
                              &pa[k * plen]);
+            });
             const int st = solve_nodes(members, pa, sp.tolerance, kept);
             if (st) return st;
             fold(members, kept);
@@ -558,9 +635,10 @@ struct Run {
             for (size_t i = 0; i < seq.size(); ++i)
                 if (wave[i] == w) members.push_back(seq[i]);
             pa.assign(members.size() * plen, 0.0);
-            for (size_t k = 0; k < members.size(); ++k)
+            workers.run((int64_t)members.size(), [&](int64_t k) {
                 initial_warp(nodes, &nodes[2 * members[k]], trusted(members[k], skip.data()), czn.data(), P.data(),
                              plen, &pa[k * plen]);
+            });
             const int st = solve_nodes(members, pa, sp.tolerance, kept);
             if (st) return st;
             for (size_t k = 0; k < members.size(); ++k) settle(members[k], kept[k]);
@@ -610,22 +688,39 @@ struct Run {
             r[i] = std::sqrt(disp[2 * i] * disp[2 * i] + disp[2 * i + 1] * disp[2 * i + 1]);
             if (fl[i] < flq - 1.5 * fiqr) flagged[i] = 1;
         }
-        std::vector<int64_t> local;
-        std::vector<double> lr;
-        for (int64_t e = 0; e < t->n_el; ++e) {
-            local.clear();
-            for (int k = 0; k < 3; ++k)
-                local.insert(local.end(), t->full.begin(t->at(e, k)), t->full.end(t->at(e, k)));
-            std::sort(local.begin(), local.end());
-            local.erase(std::unique(local.begin(), local.end()), local.end());
-            lr.clear();
-            for (int64_t q : local) lr.push_back(r[q]);
-            double lq, uq;
-            quartiles(lr, lq, uq);
-            if (lr.empty()) continue;
-            const double fence = uq + 4.0 * (uq - lq);
-            for (int64_t q : local)
-                if (r[q] > fence) rflag[q] = 1;
+        // per element (independent of each other): host threads over
+        // element ranges, each marking its own flags, merged afterwards
+        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(
+            std::min<int64_t>(16, (int64_t)std::thread::hardware_concurrency()), t->n_el / 2048));
+        std::vector<std::vector<uint8_t>> part(nth, std::vector<uint8_t>(nth > 1 ? n : 0, 0));
+        auto elems = [&](int w) {
+            std::vector<uint8_t>& mark = nth > 1 ? part[w] : rflag;
+            std::vector<int64_t> local;
+            std::vector<double> lr;
+            for (int64_t e = t->n_el * w / nth; e < t->n_el * (w + 1) / nth; ++e) {
+                local.clear();
+                for (int k = 0; k < 3; ++k)
+                    local.insert(local.end(), t->full.begin(t->at(e, k)), t->full.end(t->at(e, k)));
+                std::sort(local.begin(), local.end());
+                local.erase(std::unique(local.begin(), local.end()), local.end());
+                lr.clear();
+                for (int64_t q : local) lr.push_back(r[q]);
+                double lq, uq;
+                quartiles(lr, lq, uq);
+                if (lr.empty()) continue;
+                const double fence = uq + 4.0 * (uq - lq);
+                for (int64_t q : local)
+                    if (r[q] > fence) mark[q] = 1;
+            }
+        };
+        if (nth == 1) {
+            elems(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (int w = 0; w < nth; ++w) pool.emplace_back(elems, w);
+            for (auto& th : pool) th.join();
+            for (int w = 0; w < nth; ++w)
+                for (int64_t i = 0; i < n; ++i) rflag[i] |= part[w][i];
         }
         std::vector<int64_t> ids;
         for (int64_t i = 0; i < n; ++i)
