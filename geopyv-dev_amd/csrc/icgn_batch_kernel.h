@@ -615,11 +615,6 @@ __device__ __forceinline__ void chol_solve_neg_rd(const double* L, const double*
 // each (lane j < M: b_j in, x_j out): the forward and backward
 // substitutions go column by column, each step broadcasting one solved
 // component with two readlanes, so the registers used do not grow with M.
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
-}
 
 template <int M>
 __device__ __forceinline__ double chol_solve_neg_lanes(const double* L, const double* rd, double b, int lane) {
@@ -637,99 +632,6 @@ __device__ __forceinline__ double chol_solve_neg_lanes(const double* L, const do
         if (lane < i) z = fma(-L[i * (i + 1) / 2 + lane], xi, z);
     }
     return -z;
-}
-
-// Order-2 composition W(p)·W(dp)⁻¹ (subset.rs:547-611) spread over the lead
-// wavefront.  The warp is read from rows 3 and 4 of the product only
-// (subset.rs:596-609), and those rows are X·W(dp)⁻¹ with X rows 3 and 4 of
-// W(p): so lanes 8r + c (r < 6, c < 8) hold the augmented matrix
-// [W(dp)ᵀ | Xᵀ] and Gauss-Jordan elimination with partial pivoting (one
-// column per step: the pivot found by readlanes, rows exchanged and
-// eliminated by shuffles) leaves the two rows in columns 6 and 7.  Each lane
-// builds its own entry of the lifted matrices (the s-terms of
-// subset.rs:547-571), so no lane holds a matrix.  p and dp are read
-// lane-invariantly (LDS); false when W(dp) is singular.
-__device__ __forceinline__ double lifted6_entry(const double* q, int e) {
-    const double u = q[0], v = q[1], ux = q[2], vx = q[3], uy = q[4], vy = q[5];
-    const double uxx = q[6], vxx = q[7], uxy = q[8], vxy = q[9], uyy = q[10], vyy = q[11];
-    double w = 0.0;
-    w = e == 0 ? 1.0 + (2.0 * ux + ux * ux + u * uxx) : w;
-    w = e == 1 ? 2.0 * u * uxy + 2.0 * (1.0 + ux) * uy : w;
-    w = e == 2 ? uy * uy + u * uyy : w;
-    w = e == 3 ? 2.0 * u * (1.0 + ux) : w;
-    w = e == 4 ? 2.0 * u * uy : w;
-    w = e == 5 ? u * u : w;
-    w = e == 6 ? 0.5 * (v * uxx + 2.0 * (1.0 + ux) * vx + u * vxx) : w;
-    w = e == 7 ? 1.0 + (uy * vx + ux * vy + v * uxy + u * vxy + vy + ux) : w;
-    w = e == 8 ? 0.5 * (v * uyy + 2.0 * uy * (1.0 + vy) + u * vyy) : w;
-    w = e == 9 ? v + v * ux + u * vx : w;
-    w = e == 10 ? u + v * uy + u * vy : w;
-    w = e == 11 ? u * v : w;
-    w = e == 12 ? vx * vx + v * vxx : w;
-    w = e == 13 ? 2.0 * v * vxy + 2.0 * vx * (1.0 + vy) : w;
-    w = e == 14 ? 1.0 + (2.0 * vy + vy * vy + v * vyy) : w;
-    w = e == 15 ? 2.0 * v * vx : w;
-    w = e == 16 ? 2.0 * v * (1.0 + vy) : w;
-    w = e == 17 ? v * v : w;
-    w = e == 18 ? 0.5 * uxx : w;
-    w = e == 19 ? uxy : w;
-    w = e == 20 ? 0.5 * uyy : w;
-    w = e == 21 ? 1.0 + ux : w;
-    w = e == 22 ? uy : w;
-    w = e == 23 ? u : w;
-    w = e == 24 ? 0.5 * vxx : w;
-    w = e == 25 ? vxy : w;
-    w = e == 26 ? 0.5 * vyy : w;
-    w = e == 27 ? vx : w;
-    w = e == 28 ? 1.0 + vy : w;
-    w = e == 29 ? v : w;
-    w = e == 35 ? 1.0 : w;
-    return w;
-}
-
-__device__ __forceinline__ bool compose12_lanes(const double* p, const double* dp, int lane, double* out) {
-    const int r = lane >> 3, c = lane & 7;
-    const bool live = lane < 48;
-    // G[r][c] = W(dp)[c][r] for c < 6; X[0][r], X[1][r] (= W(p)[18 + r], W(p)[24 + r]) for c = 6, 7
-    double v = 0.0;
-    if (live) v = c < 6 ? lifted6_entry(dp, 6 * c + r) : lifted6_entry(p, (c == 6 ? 18 : 24) + r);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        int piv = k;
-        double best = fabs(readlane_f64(v, 8 * k + k));
-#pragma unroll
-        for (int i = k + 1; i < 6; ++i) {
-            const double a = fabs(readlane_f64(v, 8 * i + k));
-            if (a > best) {
-                best = a;
-                piv = i;
-            }
-        }
-        if (!(best > 0.0)) return false;
-        if (piv != k) {
-            const int src = ((r == k) ? piv : (r == piv ? k : r)) * 8 + c;
-            const double t = __shfl(v, live ? src : lane, 64);
-            v = live ? t : v;
-        }
-        const double rp = 1.0 / readlane_f64(v, 8 * k + k);
-        if (live && r == k) v *= rp;
-        const double f = __shfl(v, live ? 8 * r + k : lane, 64);  // G[r][k]
-        const double g = __shfl(v, live ? 8 * k + c : lane, 64);  // normalised G[k][c]
-        if (live && r != k) v = fma(-f, g, v);
-    }
-    out[0] = readlane_f64(v, 46);
-    out[1] = readlane_f64(v, 47);
-    out[2] = readlane_f64(v, 30) - 1.0;
-    out[3] = readlane_f64(v, 31);
-    out[4] = readlane_f64(v, 38);
-    out[5] = readlane_f64(v, 39) - 1.0;
-    out[6] = readlane_f64(v, 6) * 2.0;
-    out[7] = readlane_f64(v, 7) * 2.0;
-    out[8] = readlane_f64(v, 14);
-    out[9] = readlane_f64(v, 15);
-    out[10] = readlane_f64(v, 22) * 2.0;
-    out[11] = readlane_f64(v, 23) * 2.0;
-    return true;
 }
 
 // Order-1 composition W(p)·W(dp)⁻¹ (subset.rs:499-528) with one reciprocal

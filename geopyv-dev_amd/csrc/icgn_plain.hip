@@ -118,6 +118,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
     __shared__ double sHi[M * M];
     __shared__ double sScr[144];
     __shared__ double sP[12];
+    __shared__ double sDp[12];
     __shared__ int sCtl[4];  // [0] stop flag, [1] status, [2] iterations, [3] converged
     __shared__ double sRes[4];  // last zncc, znssd, norm
 
@@ -228,9 +229,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
                 }
         }
         __syncthreads();
-        if (lane == 0) sCtl[1] = inverse_ref<M>(sH, sHi) ? 0 : DIC_ERR_SINGULAR_HESSIAN;
-        __syncthreads();
-        if (sCtl[1] != 0) {
+        if (!inverse_ref_lanes<M>(sH, sHi, lane)) {
             if (lane == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
             return;
         }
@@ -326,19 +325,28 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
         for (int j = 0; j < M; ++j) gz[j] = wave_sum_u(gz[j]);
         zacc = wave_sum_u(zacc);
         __syncthreads();
+        // the update (subset.rs:1146-1160): the Hessian inverse (FAGN), dp
+        // one component per lane, the order-2 ICGN composition over lanes;
+        // then lane 0 keeps the books
+        int status = 0;
+        if (A.method == DIC_METHOD_FAGN && !inverse_ref_lanes<M>(sH, sHi, lane)) status = DIC_ERR_SINGULAR_HESSIAN;
+        double pn[12];
+        if (!status) {
+            if (lane < M) {
+                double acc2 = gz[0] * sHi[lane * M];
+                for (int k = 1; k < M; ++k) acc2 = acc2 + gz[k] * sHi[lane * M + k];
+                sDp[lane] = (A.method == DIC_METHOD_ICGN) ? -acc2 : acc2;
+            }
+            __syncthreads();
+            if (M == 12 && A.method == DIC_METHOD_ICGN && !compose12_ref_lanes(p, sDp, pn, sScr, lane))
+                status = DIC_ERR_SINGULAR_WARP;
+        }
         if (lane == 0) {
-            int status = 0;
-            double dp[12];
-            if (A.method == DIC_METHOD_FAGN && !inverse_ref<M>(sH, sHi)) status = DIC_ERR_SINGULAR_HESSIAN;
             if (!status) {
-                for (int r = 0; r < M; ++r) {
-                    double acc2 = gz[0] * sHi[r * M];
-                    for (int k = 1; k < M; ++k) acc2 = acc2 + gz[k] * sHi[r * M + k];
-                    dp[r] = (A.method == DIC_METHOD_ICGN) ? -acc2 : acc2;
-                }
-                double pn[12];
+                double dp[12];
+                for (int k = 0; k < M; ++k) dp[k] = sDp[k];
                 if (A.method == DIC_METHOD_ICGN) {
-                    if (!compose_icgn_dev<M>(p, dp, pn, sScr)) status = DIC_ERR_SINGULAR_WARP;
+                    if (M != 12 && !compose_icgn_dev<M>(p, dp, pn, sScr)) status = DIC_ERR_SINGULAR_WARP;
                 } else {
                     for (int k = 0; k < M; ++k) pn[k] = p[k] + dp[k];
                 }
