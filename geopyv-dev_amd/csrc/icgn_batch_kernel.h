@@ -756,7 +756,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     constexpr int NL = M * (M + 1) / 2;
     constexpr int K = Modes<M>::K;
     constexpr int NA = 3 + M;                          // per-iteration sums
-    constexpr int NH1 = (M <= 6) ? NL : 0;             // Hessian entries summed in the first setup pass
+    constexpr int NH1 = (M <= 2) ? NL : 0;             // Hessian entries summed in the first setup pass
     constexpr int V1 = 2 + 2 * M + NH1;                // first setup pass
     constexpr int V2 = 4 + 4 * K;                      // second setup pass
     GEODIC_DYN_LDS(float, lds);
@@ -767,7 +767,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     __shared__ double sRd[M];
     __shared__ double sAS[2 * M];               // A_j, S_j
     __shared__ float sDg[4 + 4 * Modes<M>::K];  // mx, my, 1/sdx, 1/sdy, rs, 1/rv
-    __shared__ double sP[12];                   // the current warp (order 2: the only copy)
+    __shared__ double sP[12];                   // the current warp (its only copy)
+    __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
     __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
     __shared__ double sG[3];  // g_m, Δg, norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
@@ -927,6 +928,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                     }
             }
         }
+        if constexpr (M == 6) hessian_rows<M, P, WPS, 0, 6>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
         if constexpr (M == 12) {
             hessian_rows<M, P, WPS, 0, 2>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
             hessian_rows<M, P, WPS, 2, 5>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
@@ -948,8 +950,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             const Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
             if (lane == 0) {
                 sPlan = pl;
-                if constexpr (M == 12)
-                    for (int k = 0; k < M; ++k) sP[k] = p0[k];
+                for (int k = 0; k < M; ++k) sP[k] = p0[k];
             }
         }
 
@@ -1021,16 +1022,16 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         return;
     }
 
-    const double size = sqrt(dn);
-    const double rn = 1.0 / dn;
-    const double max_norm2 = A.max_norm * A.max_norm;
-    const float f_m32 = uni((float)f_m);
-    // the current warp; for order 2 it lives in LDS (sP) only
-    double p[M == 12 ? 1 : M];
-    if constexpr (M != 12) {
-#pragma unroll
-        for (int k = 0; k < M; ++k) p[k] = uni(A.p0[12 * s + k]);
+    // the per-subset constants and the current warp (sP) stay in LDS
+    // through the iterations, out of the registers the pixel loop needs
+    if (tid == 0) {
+        sK[0] = f_m;
+        sK[1] = delta_f;
+        sK[2] = sa;
+        sK[3] = 1.0 / dn;
+        sK[4] = sqrt(dn);
     }
+    const float f_m32 = uni((float)f_m);
 
     int it_done = 0;
     bool converged = false;
@@ -1096,6 +1097,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         // the lead wavefront alone runs the update; the others wait at the
         // barrier below, leaving their SIMD slots to other workgroups
         if (block_sum_lead<WPS, NA>(sum, sRed, lane, wave)) {
+            const double f_m = sK[0], delta_f = sK[1], sa = sK[2], rn = sK[3], size = sK[4];
+            const double max_norm2 = A.max_norm * A.max_norm;
             // g' above is g - (float)f_m; fold the rounding of the shift back in
             const double shift = (double)f_m32 - f_m;
             const double s1 = sum[0] + dn * shift;
@@ -1131,6 +1134,9 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
                 double dp[M];
                 chol_solve_neg_rd<M>(sL, sRd, gz, dp);
+                double p[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) p[k] = sP[k];
                 if constexpr (M == 6)
                     ok = compose6_fast(p, dp, pn);
                 else
@@ -1175,10 +1181,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             status = sCtl[0];
             break;
         }
-        if constexpr (M != 12) {
-#pragma unroll
-            for (int k = 0; k < M; ++k) p[k] = uni(sP[k]);
-        }
         it_done = it;
         if (sCtl[1]) {
             converged = true;
@@ -1202,9 +1204,9 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     // on g at the last iteration's warp (its plan and staged window are still
     // current); with no iteration g is all zero and the reference's NaN
     // propagates alike ----
-    const float rf = uni((float)(1.0 / delta_f));
+    const float rf = uni((float)(1.0 / sK[1]));
     const float rg = it_done ? uni((float)(1.0 / delta_g)) : __int_as_float(0x7fc00000);
-    const float mg32 = it_done ? uni((float)(g_m - f_m)) : -f_m32;
+    const float mg32 = it_done ? uni((float)(g_m - sK[0])) : -f_m32;
     float fin[2 + 2 * K];  // Σe², Σe, Σ (z·mode - mean)·e
 #pragma unroll
     for (int j = 0; j < 2 + 2 * K; ++j) fin[j] = 0.f;
@@ -1273,7 +1275,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             ev += b * b;
         }
         for (int k = 0; k < 12; ++k) {
-            const double pk = k < M ? (M == 12 ? sP[k] : p[k]) : 0.0;
+            const double pk = k < M ? sP[k] : 0.0;
             R->p[k] = pk;
             A.p_out[12 * s + k] = pk;
         }
