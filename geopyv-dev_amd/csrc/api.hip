@@ -112,6 +112,7 @@ void dic_engine_destroy(dic_engine* d) {
     (void)hipSetDevice(e->device);
     (void)hipStreamSynchronize(e->stream);
     for (auto& pr : e->twiddles) (void)hipFree(pr.second);
+    for (auto& pr : e->fir_taps) (void)hipFree(pr.second);
     if (e->pre_tmp) (void)hipFree(e->pre_tmp);
     if (e->pix_stage) (void)hipFree(e->pix_stage);
     (void)hipEventDestroy(e->ev_pre0);
@@ -134,6 +135,14 @@ int dic_engine_sync(dic_engine* d) {
     CHECK_ARG(d, "engine is NULL");
     DIC_HIP_TRY(hipSetDevice(d->e.device));
     DIC_HIP_TRY(hipStreamSynchronize(d->e.stream));
+    return DIC_OK;
+}
+
+int dic_engine_set_prefilter(dic_engine* d, int method) {
+    CHECK_ARG(d, "engine is NULL");
+    CHECK_ARG(method == DIC_PREFILTER_FIR || method == DIC_PREFILTER_FFT, "invalid input: prefilter method");
+    std::lock_guard<std::mutex> lk(d->e.mu);
+    d->e.prefilter_method = method;
     return DIC_OK;
 }
 

@@ -39,6 +39,9 @@ struct Engine {
     std::mutex mu;
     // cached twiddle tables per FFT length (device, complex double)
     std::vector<std::pair<int, double2*>> twiddles;
+    // cached circular-FIR taps per line length (device)
+    std::vector<std::pair<int, double*>> fir_taps;
+    int prefilter_method = DIC_PREFILTER_FIR;
     // prefilter intermediate (row-pass output), grown on demand
     double* pre_tmp = nullptr;
     size_t pre_tmp_bytes = 0;

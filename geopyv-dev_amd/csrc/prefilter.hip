@@ -24,7 +24,28 @@
 //    it fetches are shared by the four neighbouring pairs in its own L2;
 //  * the column pass writes C in FP64 and the FP32 copy the batched engine
 //    reads, in the same kernel.
+//
+// The default path (DIC_PREFILTER_FIR) applies the same spectral division
+// in its spatial form.  The inverse of the circulant k is itself circulant:
+// its taps h_n[k] = (1/n) Σ_f cos(2πfk/n) / K(2πf/n) are the inverse DFT of
+// 1/K, computed once per line length on the host in long double.  They fall
+// by a factor ~0.43 per tap (the larger pole of the quintic B-spline,
+// -0.4306), to ~1e-19 at |k| = 52, so each line is a 105-tap circular FIR
+// whose dropped taps weigh less than 1e-15 of a pixel; a line shorter than
+// 105 keeps one tap per residue and is the exact inverse.  That is ~40 %
+// fewer FP64 operations than two transforms of lengths with prime factors
+// 29 or 47 and needs no barriers between stages:
+//  * rows: a workgroup stages 4 padded rows x 623 samples (replicate
+//    padding and the circular wrap resolved while loading) in LDS, stored
+//    phase-major so that each thread's 8 consecutive outputs read their
+//    window without bank conflicts; the window rolls through registers;
+//  * columns: each thread produces 16 consecutive outputs of one column from
+//    a 127-row window read straight from the row-pass output (64 lanes = 64
+//    adjacent columns, so every load is a 512-byte line), with workgroups
+//    banded by column per XCD so the overlapping windows meet in one L2.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -183,6 +204,181 @@ __global__ void __launch_bounds__(1024) prefilter_cols_kernel(const double* __re
     }
 }
 
+// ---- circular FIR form ----
+constexpr int kFirHalf = 52;                         // taps k = -52..52
+constexpr int kFirTaps = 112;                        // tap slots (105 used, zero-padded to whole chunks)
+constexpr int kChunk = 8;                            // window samples per chunk (loads in flight together)
+constexpr int kRowR = 8;                             // row pass: outputs per thread
+constexpr int kRowRows = 4;                          // rows per workgroup, one wavefront each
+constexpr int kRowTile = 64 * kRowR;                 // outputs per row per workgroup
+constexpr int kRowChunks = (kRowR + kFirTaps - 1 + kChunk - 1) / kChunk;  // 15
+constexpr int kRowWin = kRowTile - kRowR + kRowChunks * kChunk;          // 624 samples per row
+constexpr int kRowS = kRowWin / kRowR;               // samples per phase (78)
+constexpr int kColR = 16;                            // column pass: outputs per thread
+constexpr int kColRows = 4 * kColR;                  // rows per workgroup (4 wavefronts)
+constexpr int kColChunks = (kColR + kFirTaps - 1 + kChunk - 1) / kChunk; // 16
+constexpr int kTapPad = 16;                          // zero taps before slot 0 (covers t = m - r >= -15)
+constexpr int kTapLds = kTapPad + kColChunks * kChunk;                   // 144
+
+// Taps in LDS at slot t + kTapPad, zero outside [0, kFirTaps): a chunk's
+// products then need no bounds tests.
+__device__ __forceinline__ void stage_taps(double* stap, const double* __restrict__ taps) {
+    for (int i = threadIdx.x; i < kTapLds; i += blockDim.x) {
+        const int t = i - kTapPad;
+        stap[i] = (t >= 0 && t < kFirTaps) ? taps[t] : 0.0;
+    }
+}
+
+// Row pass: out[i][j] = Σ_t taps[t] P[i][(j - k1 + t) mod wp], P the
+// replicate-padded image (image.rs:436-443).
+template <typename PixT>
+__global__ void __launch_bounds__(256) fir_rows_kernel(const PixT* __restrict__ pix, int64_t h, int64_t w, int border,
+                                                       int64_t hp, int64_t wp, int k1,
+                                                       const double* __restrict__ taps, double* __restrict__ out) {
+    __shared__ double win[kRowRows][kRowR * kRowS];
+    __shared__ double stap[kTapLds];
+    const int lane = threadIdx.x & 63, rr = threadIdx.x >> 6;
+    const int64_t row = (int64_t)blockIdx.y * kRowRows + rr;
+    const int64_t c0 = (int64_t)blockIdx.x * kRowTile;
+    stage_taps(stap, taps);
+    double* wr = win[rr];
+    if (row < hp) {
+        const int64_t si = min(max(row - border, (int64_t)0), h - 1);
+        const PixT* src = pix + si * w;
+        // padded column c0 - k1 + q lies in [-k1, wp + kRowWin): one wrap
+        // either way when the row is at least as long as the window
+        const bool one_wrap = wp >= kRowWin;
+        for (int q = lane; q < kRowWin; q += 64) {
+            int64_t j = c0 - k1 + q;
+            if (one_wrap) {
+                j = j < 0 ? j + wp : (j >= wp ? j - wp : j);
+            } else {
+                j %= wp;
+                if (j < 0) j += wp;
+            }
+            const int64_t sj = min(max(j - border, (int64_t)0), w - 1);
+            wr[(q % kRowR) * kRowS + q / kRowR] = (double)src[sj];
+        }
+    }
+    __syncthreads();
+    if (row >= hp) return;
+    // window sample lane*R + m sits at phase m % R, slot lane + m / R; with
+    // R = chunk, chunk c's sample u is at phase u, slot lane + c
+    double acc[kRowR];
+#pragma unroll
+    for (int r = 0; r < kRowR; ++r) acc[r] = 0.0;
+    const double* wl = wr + lane;
+#pragma unroll 1
+    for (int c = 0; c < kRowChunks; ++c) {
+        double v[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) v[u] = wl[u * kRowS + c];
+        const double* tp = stap + kTapPad + c * kChunk;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int r = 0; r < kRowR; ++r) acc[r] = fma(tp[u - r], v[u], acc[r]);
+    }
+    double* o = out + row * wp + c0 + (int64_t)lane * kRowR;
+    const int64_t left = wp - c0 - (int64_t)lane * kRowR;
+#pragma unroll
+    for (int r = 0; r < kRowR; ++r)
+        if (r < left) o[r] = acc[r];
+}
+
+// Column pass over the row-pass output, writing C in FP64 and FP32.  Tile
+// t = (column band, row block), row blocks fastest, tiles dealt to XCDs in
+// contiguous runs.  Window row m of a thread is (i0 - k1 + m) mod hp, at
+// most one wrap when the column is at least as long as the window (else a
+// remainder), computed per row so a chunk's loads are independent.
+__global__ void __launch_bounds__(256) fir_cols_kernel(const double* __restrict__ tmp, int64_t hp, int64_t wp, int k1,
+                                                       const double* __restrict__ taps, int row_tiles, int n_tiles,
+                                                       int per_xcd, double* __restrict__ c64, float* __restrict__ c32,
+                                                       int64_t wp32) {
+    __shared__ double stap[kTapLds];
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * per_xcd + (b >> 3);
+    if (tile >= n_tiles) return;
+    stage_taps(stap, taps);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t c = (int64_t)(tile / row_tiles) * 64 + lane;
+    const int64_t i0 = (int64_t)(tile % row_tiles) * kColRows + (int64_t)wave * kColR;
+    if (i0 >= hp) return;
+    const int64_t cl = min(c, wp - 1);  // lanes past the last column load a valid one and store nothing
+    int64_t q0 = (i0 - k1) % hp;
+    if (q0 < 0) q0 += hp;
+    const bool one_wrap = hp >= kColChunks * kChunk;
+    const double* col = tmp + cl;
+    double acc[kColR];
+#pragma unroll
+    for (int r = 0; r < kColR; ++r) acc[r] = 0.0;
+#pragma unroll 1
+    for (int ch = 0; ch < kColChunks; ++ch) {
+        double v[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            int64_t q = q0 + ch * kChunk + u;
+            q = one_wrap ? (q >= hp ? q - hp : q) : q % hp;
+            v[u] = col[q * wp];
+        }
+        const double* tp = stap + kTapPad + ch * kChunk;
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int r = 0; r < kColR; ++r) acc[r] = fma(tp[u - r], v[u], acc[r]);
+    }
+    if (c >= wp) return;
+#pragma unroll
+    for (int r = 0; r < kColR; ++r) {
+        const int64_t i = i0 + r;
+        if (i < hp) {
+            c64[i * wp + c] = acc[r];
+            c32[i * wp32 + c] = (float)acc[r];
+        }
+    }
+}
+
+// Taps of the circular FIR for line length n (device, kFirTaps doubles, slot
+// t holding offset t - k1), cached per length like the twiddles.
+int get_fir_taps(Engine* e, int n, const double** out, int* k1_out) {
+    for (auto& pr : e->fir_taps)
+        if (pr.first == n) {
+            *out = pr.second;
+            *k1_out = n >= 2 * kFirHalf + 1 ? kFirHalf : n / 2;
+            return DIC_OK;
+        }
+    const int k1 = n >= 2 * kFirHalf + 1 ? kFirHalf : n / 2;
+    const int nt = n >= 2 * kFirHalf + 1 ? 2 * kFirHalf + 1 : n;  // one tap per residue when short
+    const int kmax = std::max(k1, nt - 1 - k1);
+    // h[k] = (1/n) Σ_f cos(kθ_f) / K(θ_f), cos(kθ) by the Chebyshev recurrence
+    std::vector<long double> hk(kmax + 1, 0.0L);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int f = 0; f < n; ++f) {
+        const long double c1 = cosl(two_pi * (long double)f / (long double)n);
+        const long double K = 11.0L / 20.0L + (13.0L / 30.0L) * c1 + (1.0L / 60.0L) * (2.0L * c1 * c1 - 1.0L);
+        const long double rk = 1.0L / K;
+        long double cm = 1.0L, cc = c1;
+        hk[0] += rk;
+        for (int k = 1; k <= kmax; ++k) {
+            hk[k] += cc * rk;
+            const long double nx = 2.0L * c1 * cc - cm;
+            cm = cc;
+            cc = nx;
+        }
+    }
+    std::vector<double> h(kFirTaps, 0.0);
+    for (int t = 0; t < nt; ++t) h[t] = (double)(hk[std::abs(t - k1)] / (long double)n);
+    double* d = nullptr;
+    DIC_HIP_TRY(hipMalloc(&d, sizeof(double) * kFirTaps));
+    DIC_HIP_TRY(hipMemcpyAsync(d, h.data(), sizeof(double) * kFirTaps, hipMemcpyHostToDevice, e->stream));
+    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->fir_taps.push_back({n, d});
+    *out = d;
+    *k1_out = k1;
+    return DIC_OK;
+}
+
 int threads_for(int n) {
     int t = 64;
     while (t * kMaxOut < n && t < 1024) t *= 2;
@@ -271,7 +467,7 @@ __global__ void __launch_bounds__(256) loader_convert_kernel(const uint8_t* __re
 
 int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own) {
     const int64_t h = img->h, w = img->w, hp = img->hp, wp = img->wp;
-    if (wp > (int64_t)kMaxOut * 1024 || hp > (int64_t)kMaxOut * 1024) {
+    if (e->prefilter_method == DIC_PREFILTER_FFT && (wp > (int64_t)kMaxOut * 1024 || hp > (int64_t)kMaxOut * 1024)) {
         set_error("prefilter: padded line longer than 8192 samples");
         return DIC_ERR_INVALID_INPUT;
     }
@@ -296,15 +492,26 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
         DIC_HIP_TRY(hipMalloc(&e->pre_tmp, tmp_bytes));
         e->pre_tmp_bytes = tmp_bytes;
     }
+    const bool fft = e->prefilter_method == DIC_PREFILTER_FFT;
     const double2 *tw_r = nullptr, *tw_c = nullptr;
-    int st = get_twiddles(e, (int)wp, &tw_r);
-    if (st) return st;
-    st = get_twiddles(e, (int)hp, &tw_c);
-    if (st) return st;
-    FftStages sr = make_stages((int)wp), sc = make_stages((int)hp);
-    if (sr.nst > kMaxStages || sc.nst > kMaxStages) {
-        set_error("prefilter: too many FFT stages");
-        return DIC_ERR_INVALID_INPUT;
+    const double *taps_r = nullptr, *taps_c = nullptr;
+    int k1_r = 0, k1_c = 0;
+    FftStages sr{}, sc{};
+    int st = DIC_OK;
+    if (fft) {
+        st = get_twiddles(e, (int)wp, &tw_r);
+        if (!st) st = get_twiddles(e, (int)hp, &tw_c);
+        if (st) return st;
+        sr = make_stages((int)wp);
+        sc = make_stages((int)hp);
+        if (sr.nst > kMaxStages || sc.nst > kMaxStages) {
+            set_error("prefilter: too many FFT stages");
+            return DIC_ERR_INVALID_INPUT;
+        }
+    } else {
+        st = get_fir_taps(e, (int)wp, &taps_r, &k1_r);
+        if (!st) st = get_fir_taps(e, (int)hp, &taps_c, &k1_c);
+        if (st) return st;
     }
     const int tr = threads_for((int)wp), tc = threads_for((int)hp);
     const size_t lr = sizeof(double2) * wp, lc = sizeof(double2) * hp;
@@ -325,6 +532,27 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
         DIC_HIP_TRY(hipGetLastError());
         dpix = img->grey_dev;
         pix_type = DIC_PIX_U8;
+    }
+    if (!fft) {
+        const dim3 rgrid((unsigned)((wp + kRowTile - 1) / kRowTile), (unsigned)((hp + kRowRows - 1) / kRowRows));
+        if (pix_type == DIC_PIX_U8)
+            fir_rows_kernel<uint8_t><<<rgrid, 256, 0, e->stream>>>((const uint8_t*)dpix, h, w, img->border, hp, wp,
+                                                                   k1_r, taps_r, tmp);
+        else
+            fir_rows_kernel<double><<<rgrid, 256, 0, e->stream>>>((const double*)dpix, h, w, img->border, hp, wp,
+                                                                  k1_r, taps_r, tmp);
+        DIC_HIP_TRY(hipGetLastError());
+        const int row_tiles = (int)((hp + kColRows - 1) / kColRows);
+        const int n_tiles = row_tiles * (int)((wp + 63) / 64);
+        const int per_xcd = (n_tiles + 7) / 8;
+        fir_cols_kernel<<<per_xcd * 8, 256, 0, e->stream>>>(tmp, hp, wp, k1_c, taps_c, row_tiles, n_tiles, per_xcd,
+                                                            img->coef64, img->coef32, img->wp32);
+        DIC_HIP_TRY(hipGetLastError());
+        if (!e->capturing) {
+            DIC_HIP_TRY(hipEventRecord(e->ev_pre1, e->stream));
+            e->pre_recorded = true;
+        }
+        return DIC_OK;
     }
     const int row_blocks = (int)((hp + 1) / 2);
     if (pix_type == DIC_PIX_U8) {

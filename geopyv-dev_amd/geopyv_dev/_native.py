@@ -32,11 +32,13 @@ MEM_HOST, MEM_DEVICE = 0, 1
 SHAPE = {"circle": 0, "square": 1, "semicircle": 2}
 METHOD = {"icgn": 0, "fagn": 1}
 PREC_PLAIN_F64, PREC_COMP_F32 = 0, 1
+PREFILTER_FIR, PREFILTER_FFT = 0, 1
 PRECOND = {"RG": 0, "layer-RG": 1}
 ERR_MESH_GENERATION = 4
 
 EXPORTED = [
     "dic_engine_create", "dic_engine_destroy", "dic_engine_sync", "dic_engine_timing", "dic_last_error",
+    "dic_engine_set_prefilter",
     "dic_engine_bytes_in_use", "dic_image_create", "dic_image_update", "dic_image_coef", "dic_image_info",
     "dic_image_destroy", "dic_mask_local", "dic_mask_update", "dic_batch_create", "dic_batch_set_p0",
     "dic_batch_solve", "dic_batch_results", "dic_batch_history", "dic_batch_subset_arrays",
@@ -124,6 +126,7 @@ def load():
         L.dic_engine_destroy.restype = None
         L.dic_engine_sync.argtypes = [_vp]
         L.dic_engine_timing.argtypes = [_vp, _dp, _dp]
+        L.dic_engine_set_prefilter.argtypes = [_vp, ctypes.c_int]
         L.dic_engine_bytes_in_use.argtypes = [_vp]
         L.dic_engine_bytes_in_use.restype = ctypes.c_int64
         L.dic_image_create.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
@@ -235,6 +238,10 @@ class Engine:
 
     def sync(self):
         check(load().dic_engine_sync(self.handle))
+
+    def set_prefilter(self, method):
+        """PREFILTER_FIR (default) or PREFILTER_FFT: how get_c is evaluated."""
+        check(load().dic_engine_set_prefilter(self.handle, int(method)))
 
     def timing(self):
         a, b = ctypes.c_double(), ctypes.c_double()

@@ -93,6 +93,16 @@ typedef struct {
 int dic_engine_create(int device, dic_engine** out);
 void dic_engine_destroy(dic_engine* e);
 int dic_engine_sync(dic_engine* e);
+/* How images are prefiltered (get_c, image.rs:252-327): both apply the
+ * spectral division by K(w) of every padded row, then every column.
+ * DIC_PREFILTER_FIR (the default) as a circular convolution with the inverse
+ * DFT of 1/K (105 taps; exact for lines shorter than that), DIC_PREFILTER_FFT
+ * by forward transform, division and inverse transform of each line, as the
+ * reference computes it.  Results agree to ~1e-13 of a grey level.  No
+ * reference counterpart: the choice of evaluation is this library's. */
+#define DIC_PREFILTER_FIR 0
+#define DIC_PREFILTER_FFT 1
+int dic_engine_set_prefilter(dic_engine* e, int method);
 /* Kernel time of the last prefilter and the last batch solve, measured with
  * HIP events on the engine's stream (milliseconds). */
 int dic_engine_timing(dic_engine* e, double* prefilter_ms, double* solve_ms);

@@ -30,8 +30,21 @@ def pair0():
     return dict(ref=ref, tar=tar, Cf=O.get_c(ref, 20), Cg=O.get_c(tar, 20), w=w)
 
 
-@pytest.mark.parametrize("shape,border", [((512, 512), 20), ((37, 53), 5), ((64, 81), 3), ((20, 25), 5)])
-def test_prefilter_matches_oracle(shape, border):
+@pytest.fixture(params=[_native.PREFILTER_FIR, _native.PREFILTER_FFT], ids=["fir", "fft"])
+def prefilter(request):
+    """Each prefilter evaluation in turn; the default (FIR) afterwards."""
+    e = gp._engine()
+    e.set_prefilter(request.param)
+    yield request.param
+    e.set_prefilter(_native.PREFILTER_FIR)
+
+
+# padded line lengths: 552 and 2134 (long lines, truncated FIR); 47, 63,
+# 70, 87, 30 and 35 (shorter than the 105 taps: one tap per residue, odd
+# and even); 104, 105 and 106 around the switch
+@pytest.mark.parametrize("shape,border", [((512, 512), 20), ((37, 53), 5), ((64, 81), 3), ((20, 25), 5),
+                                          ((64, 65), 20), ((20, 2048), 43)])
+def test_prefilter_matches_oracle(shape, border, prefilter):
     rng = np.random.default_rng(7)
     img = rng.uniform(0, 255, shape)
     I = gp.Image(image_gs=img, border=border)
@@ -40,6 +53,21 @@ def test_prefilter_matches_oracle(shape, border):
     assert C.shape == Co.shape
     np.testing.assert_allclose(C, Co, rtol=1e-8, atol=1e-8 * 255)
     assert np.max(np.abs(C - Co)) < 1e-10
+
+
+def test_prefilter_fir_equals_fft():
+    # the two evaluations of the same spectral division agree to round-off
+    rng = np.random.default_rng(11)
+    img = rng.integers(0, 256, (300, 401)).astype(np.uint8)
+    e = gp._engine()
+    out = {}
+    for m in (_native.PREFILTER_FIR, _native.PREFILTER_FFT):
+        e.set_prefilter(m)
+        out[m] = _native.DeviceImage(e, pixels=img, border=20).coef()
+    e.set_prefilter(_native.PREFILTER_FIR)
+    assert np.max(np.abs(out[_native.PREFILTER_FIR] - out[_native.PREFILTER_FFT])) < 1e-11
+    with pytest.raises(_native.GeodicError):
+        e.set_prefilter(7)
 
 
 def test_prefilter_u8_equals_f64():
