@@ -109,6 +109,51 @@ __device__ __forceinline__ void mode_row(double xi, double ze, double* mo) {
     }
 }
 
+// The order-2 update over the lanes of the wavefront (subset.rs:1146-1160):
+// the Hessian inverse (FAGN), dp one component per lane and the ICGN
+// composition, with the one-lane rounding; lane 0 then keeps the books.
+__device__ __forceinline__ void plain_update12(const PlainArgs& A, int64_t s, int it, int lane, const double* gz,
+                                            double zacc, double size, const double* p, double* sH, double* sHi,
+                                            double* sScr, double* sDp, double* sP, double* sRes, int* sCtl) {
+#pragma clang fp contract(off)
+    constexpr int M = 12;
+    int status = 0;
+    double pn[12];
+    if (A.method == DIC_METHOD_FAGN && !inverse_ref_lanes<M>(sH, sHi, lane)) status = DIC_ERR_SINGULAR_HESSIAN;
+    if (!status) {
+        if (lane < M) {
+            double acc2 = gz[0] * sHi[lane * M];
+            for (int k = 1; k < M; ++k) acc2 = acc2 + gz[k] * sHi[lane * M + k];
+            sDp[lane] = (A.method == DIC_METHOD_ICGN) ? -acc2 : acc2;
+        }
+        __syncthreads();
+        if (A.method == DIC_METHOD_ICGN && !compose12_ref_lanes(p, sDp, pn, sScr, lane)) status = DIC_ERR_SINGULAR_WARP;
+    }
+    if (lane == 0) {
+        if (!status) {
+            double dp[12];
+            for (int k = 0; k < M; ++k) dp[k] = sDp[k];
+            if (A.method == DIC_METHOD_FAGN)
+                for (int k = 0; k < M; ++k) pn[k] = p[k] + dp[k];
+            for (int k = 0; k < M; ++k) sP[k] = pn[k];
+            const double norm = convergence_norm_dev<M>(dp, size);
+            const double c_zncc = 1.0 - zacc / 2.0;
+            if (A.hist && it <= A.hist_cap) {
+                double* h = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
+                h[0] = (double)it;
+                h[1] = norm;
+                h[2] = c_zncc;
+                h[3] = zacc;
+            }
+            sRes[0] = c_zncc;
+            sRes[1] = zacc;
+            sRes[2] = norm;
+            sCtl[3] = norm < A.max_norm ? 1 : 0;
+        }
+        sCtl[1] = status;
+    }
+}
+
 template <int M>
 __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
 #pragma clang fp contract(off)
@@ -229,9 +274,18 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
                 }
         }
         __syncthreads();
-        if (!inverse_ref_lanes<M>(sH, sHi, lane)) {
-            if (lane == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
-            return;
+        if constexpr (M == 12) {
+            if (!inverse_ref_lanes<M>(sH, sHi, lane)) {
+                if (lane == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
+                return;
+            }
+        } else {
+            if (lane == 0) sCtl[1] = inverse_ref<M>(sH, sHi) ? 0 : DIC_ERR_SINGULAR_HESSIAN;
+            __syncthreads();
+            if (sCtl[1] != 0) {
+                if (lane == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
+                return;
+            }
         }
     }
 
@@ -325,49 +379,47 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
         for (int j = 0; j < M; ++j) gz[j] = wave_sum_u(gz[j]);
         zacc = wave_sum_u(zacc);
         __syncthreads();
-        // the update (subset.rs:1146-1160): the Hessian inverse (FAGN), dp
-        // one component per lane, the order-2 ICGN composition over lanes;
-        // then lane 0 keeps the books
-        int status = 0;
-        if (A.method == DIC_METHOD_FAGN && !inverse_ref_lanes<M>(sH, sHi, lane)) status = DIC_ERR_SINGULAR_HESSIAN;
-        double pn[12];
-        if (!status) {
-            if (lane < M) {
-                double acc2 = gz[0] * sHi[lane * M];
-                for (int k = 1; k < M; ++k) acc2 = acc2 + gz[k] * sHi[lane * M + k];
-                sDp[lane] = (A.method == DIC_METHOD_ICGN) ? -acc2 : acc2;
-            }
-            __syncthreads();
-            if (M == 12 && A.method == DIC_METHOD_ICGN && !compose12_ref_lanes(p, sDp, pn, sScr, lane))
-                status = DIC_ERR_SINGULAR_WARP;
-        }
-        if (lane == 0) {
-            if (!status) {
+        // the update (subset.rs:1146-1160): order 2 over the lanes of the
+        // wavefront; lower orders on lane 0 alone, their few operations
+        // costing less than the barriers a lane-parallel form needs
+        if constexpr (M == 12) {
+            plain_update12(A, s, it, lane, gz, zacc, size, p, sH, sHi, sScr, sDp, sP, sRes, sCtl);
+        } else {
+            if (lane == 0) {
+                int status = 0;
                 double dp[12];
-                for (int k = 0; k < M; ++k) dp[k] = sDp[k];
-                if (A.method == DIC_METHOD_ICGN) {
-                    if (M != 12 && !compose_icgn_dev<M>(p, dp, pn, sScr)) status = DIC_ERR_SINGULAR_WARP;
-                } else {
-                    for (int k = 0; k < M; ++k) pn[k] = p[k] + dp[k];
-                }
+                if (A.method == DIC_METHOD_FAGN && !inverse_ref<M>(sH, sHi)) status = DIC_ERR_SINGULAR_HESSIAN;
                 if (!status) {
-                    for (int k = 0; k < M; ++k) sP[k] = pn[k];
-                    const double norm = convergence_norm_dev<M>(dp, size);
-                    const double c_zncc = 1.0 - zacc / 2.0;
-                    if (A.hist && it <= A.hist_cap) {
-                        double* h = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
-                        h[0] = (double)it;
-                        h[1] = norm;
-                        h[2] = c_zncc;
-                        h[3] = zacc;
+                    for (int r = 0; r < M; ++r) {
+                        double acc2 = gz[0] * sHi[r * M];
+                        for (int k = 1; k < M; ++k) acc2 = acc2 + gz[k] * sHi[r * M + k];
+                        dp[r] = (A.method == DIC_METHOD_ICGN) ? -acc2 : acc2;
                     }
-                    sRes[0] = c_zncc;
-                    sRes[1] = zacc;
-                    sRes[2] = norm;
-                    sCtl[3] = norm < A.max_norm ? 1 : 0;
+                    double pn[12];
+                    if (A.method == DIC_METHOD_ICGN) {
+                        if (!compose_icgn_dev<M>(p, dp, pn, sScr)) status = DIC_ERR_SINGULAR_WARP;
+                    } else {
+                        for (int k = 0; k < M; ++k) pn[k] = p[k] + dp[k];
+                    }
+                    if (!status) {
+                        for (int k = 0; k < M; ++k) sP[k] = pn[k];
+                        const double norm = convergence_norm_dev<M>(dp, size);
+                        const double c_zncc = 1.0 - zacc / 2.0;
+                        if (A.hist && it <= A.hist_cap) {
+                            double* h = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
+                            h[0] = (double)it;
+                            h[1] = norm;
+                            h[2] = c_zncc;
+                            h[3] = zacc;
+                        }
+                        sRes[0] = c_zncc;
+                        sRes[1] = zacc;
+                        sRes[2] = norm;
+                        sCtl[3] = norm < A.max_norm ? 1 : 0;
+                    }
                 }
+                sCtl[1] = status;
             }
-            sCtl[1] = status;
         }
         __syncthreads();
         if (sCtl[1] != 0) {

@@ -2,6 +2,8 @@
 // usable from device code (one lane or all lanes redundantly).
 #pragma once
 
+#include <cstdint>
+
 #ifndef GEODIC_SIMT
 #include <hip/hip_runtime.h>
 #endif
@@ -14,7 +16,7 @@ namespace geodic {
 // then unit-lower and upper triangular solves against the permuted
 // identity.  Row-major; A is overwritten.  Returns false when singular.
 template <int M>
-__device__ bool inverse_ref(double* A, double* out) {
+__device__ __forceinline__ bool inverse_ref(double* A, double* out) {
 #pragma clang fp contract(off)
     if (M == 2) {
         const double det = A[0] * A[3] - A[2] * A[1];
@@ -42,8 +44,10 @@ __device__ bool inverse_ref(double* A, double* out) {
         out[8] = (A[0] * A[4] - A[3] * A[1]) / det;
         return true;
     }
-    int perm_a[M], perm_b[M];
-    int np = 0;
+    // row i was exchanged with row piv(i) >= i (4 bits each; piv(i) = i: no
+    // exchange), replayed on the identity in the same order afterwards
+    uint64_t pivots = 0;
+#pragma unroll 1
     for (int i = 0; i < M; ++i) {
         int piv = i;
         double best = fabs(A[i * M + i]);
@@ -55,11 +59,12 @@ __device__ bool inverse_ref(double* A, double* out) {
             }
         }
         const double diag = A[piv * M + i];
-        if (diag == 0.0) continue;
+        if (diag == 0.0) {
+            pivots |= (uint64_t)i << (4 * i);
+            continue;
+        }
+        pivots |= (uint64_t)piv << (4 * i);
         if (piv != i) {
-            perm_a[np] = i;
-            perm_b[np] = piv;
-            ++np;
             for (int c = 0; c < M; ++c) {
                 const double t = A[i * M + c];
                 A[i * M + c] = A[piv * M + c];
@@ -68,6 +73,7 @@ __device__ bool inverse_ref(double* A, double* out) {
         }
         const double inv_diag = 1.0 / diag;
         for (int r = i + 1; r < M; ++r) A[r * M + i] *= inv_diag;
+#pragma unroll 1
         for (int k = i + 1; k < M; ++k) {
             const double pk = A[i * M + k];
             for (int r = i + 1; r < M; ++r) A[r * M + k] = -pk * A[r * M + i] + A[r * M + k];
@@ -77,12 +83,17 @@ __device__ bool inverse_ref(double* A, double* out) {
         if (A[i * M + i] == 0.0) return false;
     for (int i = 0; i < M * M; ++i) out[i] = 0.0;
     for (int i = 0; i < M; ++i) out[i * M + i] = 1.0;
-    for (int q = 0; q < np; ++q)
-        for (int c = 0; c < M; ++c) {
-            const double t = out[perm_a[q] * M + c];
-            out[perm_a[q] * M + c] = out[perm_b[q] * M + c];
-            out[perm_b[q] * M + c] = t;
-        }
+#pragma unroll 1
+    for (int i = 0; i < M; ++i) {
+        const int piv = (int)((pivots >> (4 * i)) & 15);
+        if (piv != i)
+            for (int c = 0; c < M; ++c) {
+                const double t = out[i * M + c];
+                out[i * M + c] = out[piv * M + c];
+                out[piv * M + c] = t;
+            }
+    }
+#pragma unroll 1
     for (int c = 0; c < M; ++c) {
         for (int i = 0; i < M; ++i) {
             const double coeff = out[i * M + c];
