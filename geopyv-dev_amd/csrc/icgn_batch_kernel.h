@@ -611,27 +611,29 @@ __device__ __forceinline__ void chol_solve_neg_rd(const double* L, const double*
     for (int i = 0; i < M; ++i) x[i] = -x[i];
 }
 
-// The same solve with the lead wavefront's lanes holding one component
-// each (lane j < M: b_j in, x_j out): the forward and backward
-// substitutions go column by column, each step broadcasting one solved
-// component with two readlanes, so the registers used do not grow with M.
-
+// Column c of (LLᵀ)⁻¹ from the factor (rd = reciprocal diagonal), written to
+// Hinv (row-major): forward then backward substitution of e_c.  The factor
+// is read at lane-invariant addresses, so lanes solving different columns
+// share each LDS read.
 template <int M>
-__device__ __forceinline__ double chol_solve_neg_lanes(const double* L, const double* rd, double b, int lane) {
-    double z = b;
+__device__ __forceinline__ void chol_inverse_column(const double* L, const double* rd, int c, double* Hinv) {
+    double x[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const double yi = readlane_f64(z, i) * rd[i];
-        if (lane == i) z = yi;
-        if (lane > i && lane < M) z = fma(-L[lane * (lane + 1) / 2 + i], yi, z);
+        double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) s = fma(-L[i * (i + 1) / 2 + k], x[k], s);
+        x[i] = s * rd[i];
     }
 #pragma unroll
     for (int i = M - 1; i >= 0; --i) {
-        const double xi = readlane_f64(z, i) * rd[i];
-        if (lane == i) z = xi;
-        if (lane < i) z = fma(-L[i * (i + 1) / 2 + lane], xi, z);
+        double s = x[i];
+#pragma unroll
+        for (int k = i + 1; k < M; ++k) s = fma(-L[k * (k + 1) / 2 + i], x[k], s);
+        x[i] = s * rd[i];
     }
-    return -z;
+#pragma unroll
+    for (int i = 0; i < M; ++i) Hinv[i * M + c] = x[i];
 }
 
 // Order-1 composition W(p)·W(dp)⁻¹ (subset.rs:499-528) with one reciprocal
@@ -770,6 +772,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     __shared__ double sP[12];                   // the current warp (its only copy)
     __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
     __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
+    __shared__ double sHv[M == 12 ? M * M : 1];  // order 2: the Hessian's inverse
     __shared__ double sG[3];  // g_m, Δg, norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[2];
@@ -941,6 +944,13 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
             if (ok)
                 for (int i = 0; i < M; ++i) sRd[i] = 1.0 / sL[i * (i + 1) / 2 + i];
+        }
+        if constexpr (M == 12) {
+            // order 2: H⁻¹ = L⁻ᵀL⁻¹ once, lane c of the lead wavefront
+            // solving for column c, so each iteration's update is one dot
+            // product per lane instead of two dependent substitutions
+            __syncthreads();
+            if (wave == 0 && lane < M && sCtl[0] == 0) chol_inverse_column<M>(sL, sRd, lane, sHv);
         }
         // ... and the lead wavefront plans the first iteration
         if (wave == 0) {
@@ -1117,12 +1127,15 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             if constexpr (M == 12) {
                 // lane j solves component j; the update then sits in LDS,
                 // and the whole wavefront composes it with the warp
-                double gzl = 0.0;
+                double gz[M];
 #pragma unroll
-                for (int j = 0; j < M; ++j)
-                    if (lane == j) gzl = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
-                const double dpl = chol_solve_neg_lanes<M>(sL, sRd, gzl, lane);
-                if (lane < M) sDp[lane] = dpl;
+                for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
+                if (lane < M) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) acc = fma(sHv[lane * M + k], gz[k], acc);
+                    sDp[lane] = -acc;
+                }
                 __builtin_amdgcn_wave_barrier();
                 ok = compose12_lanes(sP, sDp, lane, pn);
 #pragma unroll
