@@ -263,8 +263,13 @@ __device__ __forceinline__ bool compose12_lanes(const double* p, const double* d
     // G[r][c] = W(dp)[c][r] for c < 6; X[0][r], X[1][r] (= W(p)[18 + r], W(p)[24 + r]) for c = 6, 7
     double v = 0.0;
     if (live) v = c < 6 ? lifted6_entry(dp, 6 * c + r) : lifted6_entry(p, (c == 6 ? 18 : 24) + r);
+    // row 5 of W(dp) is [0 0 0 0 0 1], so column 5 of G is e5 and stays so
+    // while row 5 is never exchanged: the last step then has pivot 1 and
+    // nothing to eliminate
+    bool moved5 = false;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
+        if (k == 5 && !moved5) break;
         int piv = k;
         double best = fabs(readlane_f64(v, 8 * k + k));
 #pragma unroll
@@ -276,6 +281,7 @@ __device__ __forceinline__ bool compose12_lanes(const double* p, const double* d
             }
         }
         if (!(best > 0.0)) return false;
+        moved5 = moved5 || piv == 5;
         if (piv != k) {
             const int src = ((r == k) ? piv : (r == piv ? k : r)) * 8 + c;
             const double t = __shfl(v, live ? src : lane, 64);
