@@ -824,29 +824,7 @@ void dic_frame_graph_destroy(dic_frame_graph* gr) {
 // ---------------------------------------------------------------------------
 // list solves: some subsets of a batch, each from its own initial warp
 // ---------------------------------------------------------------------------
-namespace {
-
-// p0 rows of the list into the batch's initial warps: one thread per value
-__global__ void list_scatter_kernel(const int32_t* __restrict__ idx, const double* __restrict__ src,
-                                    double* __restrict__ p0, int64_t count) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count * 12) return;
-    const int64_t r = k / 12;
-    p0[(int64_t)idx[r] * 12 + (k - r * 12)] = src[k];
-}
-
-// results of the listed subsets, in list order: one thread per 8-byte word
-__global__ void list_gather_kernel(const int32_t* __restrict__ idx, const dic_subset_result* __restrict__ res,
-                                   dic_subset_result* __restrict__ dst, int64_t count) {
-    constexpr int64_t W = sizeof(dic_subset_result) / 8;
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count * W) return;
-    const int64_t r = k / W;
-    reinterpret_cast<uint64_t*>(dst)[k] = reinterpret_cast<const uint64_t*>(res + idx[r])[k - r * W];
-}
 static_assert(sizeof(dic_subset_result) % 8 == 0, "result record must be whole 8-byte words");
-
-}  // namespace
 
 namespace geodic {
 
@@ -886,29 +864,40 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
     int32_t* h_ord = reinterpret_cast<int32_t*>(b->h_list_io + ib);
     double* h_p0 = reinterpret_cast<double*>(b->h_list_io + 2 * ib);
     const dic_subset_result* h_res = reinterpret_cast<const dic_subset_result*>(b->h_list_io + res_off);
-    std::memcpy(h_idx, idx, sizeof(int32_t) * count);
-    std::memcpy(h_ord, idx, sizeof(int32_t) * count);
-    // the batched engine's classes need the list sorted by template size
-    std::stable_sort(h_ord, h_ord + count, [&](int32_t a, int32_t c) {
-        const int ta = b->h_tpl_index[a], tc = b->h_tpl_index[c];
-        return b->h_tpl_start[ta + 1] - b->h_tpl_start[ta] > b->h_tpl_start[tc + 1] - b->h_tpl_start[tc];
-    });
-    std::memcpy(h_p0, p0, sizeof(double) * 12 * count);
     const int32_t* d_idx = reinterpret_cast<const int32_t*>(b->d_list_io);
     const int32_t* d_ord = reinterpret_cast<const int32_t*>(b->d_list_io + ib);
     const double* d_src = reinterpret_cast<const double*>(b->d_list_io + 2 * ib);
     dic_subset_result* d_res = reinterpret_cast<dic_subset_result*>(b->d_list_io + res_off);
-    const int64_t nv = count * 12;
-    list_scatter_kernel<<<(unsigned)((nv + 255) / 256), 256, 0, e->stream>>>(d_idx, d_src, b->d_p0, count);
-    DIC_HIP_TRY(hipGetLastError());
-    int st = prm->precision == DIC_PREC_PLAIN_F64 ? launch_icgn_plain(b, f, g, prm, d_idx, count)
-                                                  : launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count, true);
-    if (st) return st;
-    const int64_t nw = count * (int64_t)(sizeof(dic_subset_result) / 8);
-    list_gather_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, e->stream>>>(d_idx, b->d_res, d_res, count);
-    DIC_HIP_TRY(hipGetLastError());
-    DIC_HIP_TRY(hipStreamSynchronize(e->stream));
-    std::memcpy(out, h_res, sizeof(dic_subset_result) * count);
+    // the kernels read each listed subset's warp from the staging and write
+    // its result record there, in their dispatch order: the list's own order
+    // for the plain kernel, by template size (largest first) for the
+    // batched engine, whose size classes need that
+    int st = DIC_OK;
+    if (prm->precision == DIC_PREC_PLAIN_F64) {
+        std::memcpy(h_idx, idx, sizeof(int32_t) * count);
+        std::memcpy(h_p0, p0, sizeof(double) * 12 * count);
+        st = launch_icgn_plain(b, f, g, prm, d_idx, count, d_src, d_res);
+        if (st) return st;
+        DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+        std::memcpy(out, h_res, sizeof(dic_subset_result) * count);
+    } else {
+        std::vector<int32_t>& pos = b->list_pos;
+        pos.resize(count);
+        for (int64_t k = 0; k < count; ++k) pos[k] = (int32_t)k;
+        auto npx = [&](int32_t k) {
+            const int t = b->h_tpl_index[idx[k]];
+            return b->h_tpl_start[t + 1] - b->h_tpl_start[t];
+        };
+        std::stable_sort(pos.begin(), pos.end(), [&](int32_t a, int32_t c) { return npx(a) > npx(c); });
+        for (int64_t q = 0; q < count; ++q) {
+            h_ord[q] = idx[pos[q]];
+            std::memcpy(h_p0 + 12 * q, p0 + 12 * (int64_t)pos[q], sizeof(double) * 12);
+        }
+        st = launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count, true, d_src, d_res);
+        if (st) return st;
+        DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+        for (int64_t q = 0; q < count; ++q) out[pos[q]] = h_res[q];
+    }
     b->last_precision = prm->precision;
     b->last_method = prm->method;
     b->last_max_it = prm->max_iterations;

@@ -12,9 +12,13 @@ using batch::kRedSlots;
 
 template <int M, int P, int WPS, int S>
 int launch_mpw(const BatchArgs& A, int blocks, size_t lds, hipStream_t stream) {
-    DIC_HIP_TRY(hipFuncSetAttribute((const void*)icgn_batch_kernel<M, P, WPS, S>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    icgn_batch_kernel<M, P, WPS, S><<<blocks, 64 * WPS, lds, stream>>>(A);
+    const void* k = A.res_list ? (const void*)icgn_batch_kernel<M, P, WPS, S, true>
+                               : (const void*)icgn_batch_kernel<M, P, WPS, S, false>;
+    DIC_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (A.res_list)
+        icgn_batch_kernel<M, P, WPS, S, true><<<blocks, 64 * WPS, lds, stream>>>(A);
+    else
+        icgn_batch_kernel<M, P, WPS, S, false><<<blocks, 64 * WPS, lds, stream>>>(A);
     DIC_HIP_TRY(hipGetLastError());
     return DIC_OK;
 }
@@ -108,6 +112,8 @@ BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.p0 = b->d_p0;
     A.p_out = b->d_p;
     A.res = b->d_res;
+    A.p0_list = nullptr;
+    A.res_list = nullptr;
     A.hist = prm->want_history ? b->d_hist : nullptr;
     A.hist_cap = prm->want_history ? b->hist_cap : 0;
     A.max_it = prm->max_iterations;
@@ -123,6 +129,8 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
     }
     BatchArgs Ac = A;
     Ac.perm = (d_perm ? d_perm : b->d_order_perm) + c.start;
+    if (A.p0_list) Ac.p0_list = A.p0_list + 12 * c.start;
+    if (A.res_list) Ac.res_list = A.res_list + c.start;
     Ac.n_sub = c.count;
     Ac.win_slack = c.slack;
     // LDS: the coefficient window (template box, slack, taps, cell rounding
@@ -140,9 +148,12 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
 }  // namespace
 
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* d_perm,
-                      const int32_t* h_perm, int64_t count, bool latency) {
+                      const int32_t* h_perm, int64_t count, bool latency, const double* p0_list,
+                      dic_subset_result* res_list) {
     Engine* e = b->eng;
-    const BatchArgs A = common_args(b, f, g, prm);
+    BatchArgs A = common_args(b, f, g, prm);
+    A.p0_list = p0_list;
+    A.res_list = res_list;
     std::vector<SizeClass> cls = size_classes(b, h_perm, count);
     if (latency && b->waves_per_subset == 0)
         for (SizeClass& c : cls)

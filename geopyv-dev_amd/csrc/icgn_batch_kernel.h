@@ -87,6 +87,10 @@ struct BatchArgs {
     double max_norm;
     int win_cap;   // floats of LDS for the coefficient window (a multiple of 4)
     int win_slack; // pixels of slack around a restaged window
+    // list solves: the initial warps (12 each) and result records in
+    // dispatch order (device-mapped host memory), else null
+    const double* p0_list;
+    dic_subset_result* res_list;
 };
 
 constexpr int kRedSlots = 40;  // doubles per wavefront in the cross-wave reduction buffer
@@ -752,7 +756,10 @@ __device__ __forceinline__ Plan load_plan(const Plan* sp) {
     return P;
 }
 
-template <int M, int P, int WPS, int S>
+// LIST: a list solve, reading the initial warps from A.p0_list and writing
+// the result records to A.res_list in dispatch order (a separate
+// instantiation, so whole-batch solves compile exactly as without it)
+template <int M, int P, int WPS, int S, bool LIST = false>
 __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_batch_kernel(BatchArgs A) {
     constexpr int NT = 64 * WPS;
     constexpr int NL = M * (M + 1) / 2;
@@ -803,7 +810,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     const int bx = centre_ok ? (int)bxd : 0, by = centre_ok ? (int)byd : 0;
     const float fxc = uni((float)(xc - bxd)), fyc = uni((float)(yc - byd));
     const double dn = (double)n;
-    dic_subset_result* R = A.res + s;
+    dic_subset_result* R = LIST ? A.res_list + idx : A.res + s;
     if (tid == 0) {
         R->n_px = n;
         R->order = (M == 2) ? 0 : (M == 6 ? 1 : 2);
@@ -956,7 +963,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         if (wave == 0) {
             double p0[M];
 #pragma unroll
-            for (int k = 0; k < M; ++k) p0[k] = uni(A.p0[12 * s + k]);
+            for (int k = 0; k < M; ++k) p0[k] = uni(LIST ? A.p0_list[12 * idx + k] : A.p0[12 * s + k]);
             const Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
             if (lane == 0) {
                 sPlan = pl;

@@ -42,6 +42,8 @@ struct PlainArgs {
     int max_it;
     double tol;
     double max_norm;
+    const double* p0_list;        // with a list: warps in list order (else null)
+    dic_subset_result* res_list;  // with a list: results in list order (else null)
 };
 
 template <int M>
@@ -154,7 +156,10 @@ __device__ __forceinline__ void plain_update12(const PlainArgs& A, int64_t s, in
     }
 }
 
-template <int M>
+// LIST: a list solve with the warps in A.p0_list and the result records
+// to A.res_list in list order (a separate instantiation, so whole-batch
+// solves compile exactly as without it)
+template <int M, bool LIST = false>
 __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
 #pragma clang fp contract(off)
     constexpr int NP = M * (M + 1) / 2;
@@ -176,7 +181,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
     const int32_t* offs = A.tpl_xy + st;
     const double xc = A.coords[2 * s], yc = A.coords[2 * s + 1];
     const double dn = (double)n;
-    dic_subset_result* R = A.res + s;
+    dic_subset_result* R = LIST ? A.res_list + blockIdx.x : A.res + s;
 
     if (lane == 0) {
         for (int k = 0; k < 12; ++k) R->p[k] = 0.0;
@@ -241,8 +246,9 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
     const double size = sqrt(dn);
 
     double p[12];
+    const double* p0row = LIST ? A.p0_list + 12 * (int64_t)blockIdx.x : A.p0 + 12 * s;
 #pragma unroll
-    for (int k = 0; k < 12; ++k) p[k] = (k < M) ? A.p0[12 * s + k] : 0.0;
+    for (int k = 0; k < 12; ++k) p[k] = (k < M) ? p0row[k] : 0.0;
 
     if (A.method == DIC_METHOD_ICGN) {
         // SDI + Hessian, loop-invariant (subset.rs:1141-1143)
@@ -537,7 +543,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
 }  // namespace
 
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* list,
-                      int64_t count) {
+                      int64_t count, const double* p0_list, dic_subset_result* res_list) {
     Engine* e = b->eng;
     const int fagn = prm->method == DIC_METHOD_FAGN;
     const int arrays = fagn ? 6 : 4;
@@ -583,6 +589,8 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.scr_off = b->d_scr_off;
     A.scratch = b->d_scratch;
     A.p0 = b->d_p0;
+    A.p0_list = list ? p0_list : nullptr;
+    A.res_list = list ? res_list : nullptr;
     A.p_out = b->d_p;
     A.res = b->d_res;
     A.hist = prm->want_history ? b->d_hist : nullptr;
@@ -595,12 +603,16 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     const int order = b->max_order;
     if (A.n_sub == 0) return DIC_OK;
     const dim3 grid((unsigned)A.n_sub), block(64);
+    const bool lst = A.res_list != nullptr;
     if (order == 0)
-        icgn_plain_kernel<2><<<grid, block, 0, e->stream>>>(A);
+        lst ? icgn_plain_kernel<2, true><<<grid, block, 0, e->stream>>>(A)
+            : icgn_plain_kernel<2, false><<<grid, block, 0, e->stream>>>(A);
     else if (order == 1)
-        icgn_plain_kernel<6><<<grid, block, 0, e->stream>>>(A);
+        lst ? icgn_plain_kernel<6, true><<<grid, block, 0, e->stream>>>(A)
+            : icgn_plain_kernel<6, false><<<grid, block, 0, e->stream>>>(A);
     else
-        icgn_plain_kernel<12><<<grid, block, 0, e->stream>>>(A);
+        lst ? icgn_plain_kernel<12, true><<<grid, block, 0, e->stream>>>(A)
+            : icgn_plain_kernel<12, false><<<grid, block, 0, e->stream>>>(A);
     DIC_HIP_TRY(hipGetLastError());
     return DIC_OK;
 }

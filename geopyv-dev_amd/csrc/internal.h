@@ -121,6 +121,7 @@ struct Batch {
     char* d_list_io = nullptr;                // the same memory as the device sees it
     int64_t list_cap = 0;
     std::vector<uint32_t> list_mark;          // duplicate check
+    std::vector<int32_t> list_pos;            // list positions in dispatch order
     uint32_t list_stamp = 0;
     int last_precision = -1;
     int last_method = -1;
@@ -133,9 +134,13 @@ struct Batch {
 // engine's, for work captured into graphs
 int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own = nullptr);
 // icgn_plain.hip
-// list: device array of `count` subset indices to solve (all subsets when null)
+// list: device array of `count` subset indices to solve (all subsets when
+// null); p0_list / res_list (with a list): the listed subsets' initial warps
+// (12 each) and result records in list order, device-visible (the batch's
+// own p0 and results are then neither read nor written for them)
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
-                      const int32_t* list = nullptr, int64_t count = 0);
+                      const int32_t* list = nullptr, int64_t count = 0, const double* p0_list = nullptr,
+                      dic_subset_result* res_list = nullptr);
 // icgn_batch.hip
 struct SizeClass {
     int64_t start, count;  // run of the dispatch order
@@ -151,10 +156,11 @@ std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm = null
 // d_perm/h_perm/count: a dispatch order over some of the batch's subsets
 // (device and host copies); the whole batch in its own order when null
 // latency: a small list; each class takes more wavefronts per subset
-// while the whole list still fits on the chip at two wavefronts per SIMD
+// while the whole list still fits on the chip at two wavefronts per SIMD.
+// p0_list / res_list: as for launch_icgn_plain, in the dispatch order
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
                       const int32_t* d_perm = nullptr, const int32_t* h_perm = nullptr, int64_t count = 0,
-                      bool latency = false);
+                      bool latency = false, const double* p0_list = nullptr, dic_subset_result* res_list = nullptr);
 // times the candidate schedules of every size class and keeps the fastest
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 
