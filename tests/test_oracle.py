@@ -86,6 +86,50 @@ def test_get_c_dense_circulant_solve():
     assert np.max(np.abs(O.get_c(img, b) - C_dense)) < 1e-11
 
 
+def _fir_taps(n, half=52):
+    """Circular-FIR form of the spectral division: the inverse DFT of 1/K
+    (numpy's FFT, independent of the oracle's), taps -half..half, or one
+    tap per residue when the line is shorter than 2*half + 1."""
+    w = 2.0 * np.pi * np.arange(n) / n
+    h = np.real(np.fft.ifft(1.0 / (11 / 20 + (13 / 30) * np.cos(w) + (1 / 60) * np.cos(2 * w))))
+    k1 = half if n >= 2 * half + 1 else n // 2
+    nt = 2 * half + 1 if n >= 2 * half + 1 else n
+    return np.array([h[(t - k1) % n] for t in range(nt)]), k1
+
+
+def _fir_rows(P):
+    n = P.shape[1]
+    taps, k1 = _fir_taps(n)
+    out = np.zeros_like(P)
+    for t, v in enumerate(taps):
+        out = out + v * np.roll(P, -(t - k1), axis=1)
+    return out
+
+
+@pytest.mark.parametrize("shape,border", [((64, 65), 20), ((20, 25), 5), ((150, 40), 20)])
+def test_get_c_equals_truncated_circular_fir(shape, border):
+    """The device's default prefilter applies each line's spectral division
+    as a circular FIR with the inverse DFT of 1/K truncated to |k| <= 52
+    (taps fall by the pole 0.4306 per step): it matches the oracle's
+    FFT division to 1e-10, exactly where a line is shorter than 105 (padded
+    lengths here: 104, 105, 30, 35, 190, 80)."""
+    img = np.random.default_rng(9).uniform(0, 255, shape)
+    P = O.pad_replicate(img, border)
+    C = _fir_rows(_fir_rows(P).T).T
+    assert np.max(np.abs(C - O.get_c(img, border))) < 1e-10
+
+
+def test_fir_taps_decay_and_sum():
+    taps, k1 = _fir_taps(2088)
+    assert k1 == 52 and len(taps) == 105
+    assert abs(taps.sum() - 1.0) < 1e-12          # K(0) = 1: the inverse keeps a constant line
+    # the last kept tap is at numpy's FFT round-off (~1e-17); the true value
+    # is below 1e-18, as 0.4306**52 times the leading coefficient shows
+    assert abs(taps[k1 + 52]) < 1e-16 and abs(taps[k1 + 40]) > 1e-15
+    ratio = abs(taps[k1 + 21] / taps[k1 + 20])
+    assert abs(ratio - 0.430575347) < 1e-7     # the larger pole of the quintic B-spline
+
+
 # ---------------------------------------------------------------------------
 # eval / eval_grad (image.rs:115-230)
 # ---------------------------------------------------------------------------
