@@ -212,9 +212,10 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 // wavefront.  The warp is read from rows 3 and 4 of the product only
 // (subset.rs:596-609), and those rows are X·W(dp)⁻¹ with X rows 3 and 4 of
 // W(p): so lanes 8r + c (r < 6, c < 8) hold the augmented matrix
-// [W(dp)ᵀ | Xᵀ] and Gauss-Jordan elimination with partial pivoting (one
-// column per step: the pivot found by readlanes, rows exchanged and
-// eliminated by shuffles) leaves the two rows in columns 6 and 7.  Each lane
+// [W(dp)ᵀ | Xᵀ] and Gauss-Jordan elimination with threshold pivoting (one
+// column per step: the diagonal, or the column's largest entry found by
+// readlanes when the diagonal is small, rows exchanged and eliminated by
+// shuffles) leaves the two rows in columns 6 and 7.  Each lane
 // builds its own entry of the lifted matrices (the s-terms of
 // subset.rs:547-571), so no lane holds a matrix.  p and dp are read
 // lane-invariantly (LDS); false when W(dp) is singular.
@@ -257,12 +258,32 @@ __device__ __forceinline__ double lifted6_entry(const double* q, int e) {
     return w;
 }
 
+// Entry (r, c) of [W(dp)ᵀ | Xᵀ]: the quadratic rows 0-2 of W(dp) (the
+// s-terms) by lifted6_entry over their 18 positions only; rows 3-5 of W(dp)
+// and the two rows of W(p) in X are the warp's own terms, picked directly.
+__device__ __forceinline__ double lifted_aug_entry(const double* p, const double* dp, int r, int c) {
+#pragma clang fp contract(off)
+    if (c < 3) return lifted6_entry(dp, 6 * c + r);
+    if (c == 5) return r == 5 ? 1.0 : 0.0;
+    const double* q = c < 5 ? dp : p;
+    const bool vrow = c == 4 || c == 7;  // row 4 (the v terms), else row 3 (the u terms)
+    const double a0 = vrow ? q[7] : q[6], a1 = vrow ? q[9] : q[8], a2 = vrow ? q[11] : q[10];
+    const double a3 = vrow ? q[3] : q[2], a4 = vrow ? q[5] : q[4], a5 = vrow ? q[1] : q[0];
+    double w = 0.5 * a0;
+    w = r == 1 ? a1 : w;
+    w = r == 2 ? 0.5 * a2 : w;
+    w = r == 3 ? (vrow ? a3 : 1.0 + a3) : w;
+    w = r == 4 ? (vrow ? 1.0 + a4 : a4) : w;
+    w = r == 5 ? a5 : w;
+    return w;
+}
+
 __device__ __forceinline__ bool compose12_lanes(const double* p, const double* dp, int lane, double* out) {
     const int r = lane >> 3, c = lane & 7;
     const bool live = lane < 48;
     // G[r][c] = W(dp)[c][r] for c < 6; X[0][r], X[1][r] (= W(p)[18 + r], W(p)[24 + r]) for c = 6, 7
     double v = 0.0;
-    if (live) v = c < 6 ? lifted6_entry(dp, 6 * c + r) : lifted6_entry(p, (c == 6 ? 18 : 24) + r);
+    if (live) v = lifted_aug_entry(p, dp, r, c);
     // row 5 of W(dp) is [0 0 0 0 0 1], so column 5 of G is e5 and stays so
     // while row 5 is never exchanged: the last step then has pivot 1 and
     // nothing to eliminate
@@ -270,17 +291,23 @@ __device__ __forceinline__ bool compose12_lanes(const double* p, const double* d
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         if (k == 5 && !moved5) break;
+        // threshold pivoting: W(dp) is the identity plus terms of the order
+        // of the update, so the diagonal serves unless it falls below 1/4
+        // (multipliers then stay below 4 times the column's entries); only
+        // then is the column searched for its largest entry
         int piv = k;
         double best = fabs(readlane_f64(v, 8 * k + k));
+        if (!(best >= 0.25)) {
 #pragma unroll
-        for (int i = k + 1; i < 6; ++i) {
-            const double a = fabs(readlane_f64(v, 8 * i + k));
-            if (a > best) {
-                best = a;
-                piv = i;
+            for (int i = k + 1; i < 6; ++i) {
+                const double a = fabs(readlane_f64(v, 8 * i + k));
+                if (a > best) {
+                    best = a;
+                    piv = i;
+                }
             }
+            if (!(best > 0.0)) return false;
         }
-        if (!(best > 0.0)) return false;
         moved5 = moved5 || piv == 5;
         if (piv != k) {
             const int src = ((r == k) ? piv : (r == piv ? k : r)) * 8 + c;
