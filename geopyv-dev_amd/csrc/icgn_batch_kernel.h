@@ -93,7 +93,7 @@ struct BatchArgs {
     dic_subset_result* res_list;
 };
 
-constexpr int kRedSlots = 40;  // doubles per wavefront in the cross-wave reduction buffer
+constexpr int kRedSlots = 42;  // doubles per wavefront in the cross-wave reduction buffer
 
 template <int M>
 struct Modes {
@@ -615,6 +615,52 @@ __device__ __forceinline__ void chol_solve_neg_rd(const double* L, const double*
     for (int i = 0; i < M; ++i) x[i] = -x[i];
 }
 
+// LDS writes of some lanes made visible to the other lanes of the same
+// wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The factorisation of cholesky<M> (L packed row-major lower triangle, rd
+// = reciprocal diagonal) by the lanes of one wavefront, column by column:
+// the diagonal's square root, the column below it divided by it (lane i
+// takes row i), then the trailing block less the column's outer product
+// (one entry per lane).  Every entry meets the same subtractions in the
+// same order as in the one-lane form.  A (row-major, LDS) is overwritten;
+// every lane of the wavefront calls it; the result is the same in all.
+template <int M>
+__device__ __forceinline__ bool cholesky_lanes(double* A, double* L, double* rd, int lane) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double akk = A[k * M + k];
+        if (!(akk > 0.0)) return false;
+        const double d = sqrt(akk);
+        if (lane == 0) {
+            L[k * (k + 1) / 2 + k] = d;
+            rd[k] = 1.0 / d;
+        }
+        if (lane > k && lane < M) {
+            const double lik = A[lane * M + k] / d;
+            A[lane * M + k] = lik;
+            L[lane * (lane + 1) / 2 + k] = lik;
+        }
+        wave_lds_sync();
+        const int T = M - 1 - k;  // trailing block T x T, lower triangle used (k unrolled: a constant)
+#pragma unroll
+        for (int e0 = 0; e0 < T * T; e0 += 64) {
+            const int e = e0 + lane;
+            if (e < T * T) {
+                const int i = k + 1 + e / T, j = k + 1 + e % T;
+                if (j <= i) A[i * M + j] -= A[i * M + k] * A[j * M + k];
+            }
+        }
+        wave_lds_sync();
+    }
+    return true;
+}
+
 // Column c of (LLᵀ)⁻¹ from the factor (rd = reciprocal diagonal), written to
 // Hinv (row-major): forward then backward substitution of e_c.  The factor
 // is read at lane-invariant addresses, so lanes solving different columns
@@ -940,13 +986,19 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         }
         if constexpr (M == 6) hessian_rows<M, P, WPS, 0, 6>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
         if constexpr (M == 12) {
-            hessian_rows<M, P, WPS, 0, 2>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
-            hessian_rows<M, P, WPS, 2, 5>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
-            hessian_rows<M, P, WPS, 5, 12>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+            hessian_rows<M, P, WPS, 0, 4>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
+            hessian_rows<M, P, WPS, 4, 12>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
         }
         // the lead lane factorises (its status reaches every wavefront
         // through the barriers of the next reduction) ...
-        if (tid == 0) {
+        if constexpr (M == 12) {
+            // order 2: the lead wavefront factorises
+            __syncthreads();
+            if (wave == 0) {
+                const bool ok = cholesky_lanes<M>(sH, sL, sRd, lane);
+                if (lane == 0) sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
+            }
+        } else if (tid == 0) {
             const bool ok = cholesky<M>(sH, sL);
             sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
             if (ok)

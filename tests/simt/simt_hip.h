@@ -125,6 +125,8 @@ inline void launch(int grid, int threads, size_t dyn_lds_bytes, const std::funct
 
 inline void __syncthreads() { simt::sync(); }
 inline void __builtin_amdgcn_wave_barrier() { simt::wave_sync(); }
+// memory fence (scope ignored: every lane is a thread of this process)
+inline void __builtin_amdgcn_fence(int, const char*) { std::atomic_thread_fence(std::memory_order_seq_cst); }
 template <class T>
 T __shfl_xor(T v, int off, int = 64) {
     return simt::xchg(v, simt::t_thread.x ^ off);
