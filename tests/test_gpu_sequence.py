@@ -9,7 +9,7 @@ import pytest
 
 import oracle as O
 import geopyv_dev as gp
-from geopyv_dev import workloads
+from geopyv_dev import _native, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -22,10 +22,16 @@ def seq():
     return frames, coords, mask
 
 
-def test_graph_replay_equals_eager(seq):
+@pytest.mark.parametrize("prefilter", [_native.PREFILTER_FIR, _native.PREFILTER_FFT], ids=["fir", "fft"])
+def test_graph_replay_equals_eager(seq, prefilter):
     frames, coords, mask = seq
-    rg = gp.solve_sequence(frames, coords, mask, reference_every=3, use_graph=True)
-    re = gp.solve_sequence(frames, coords, mask, reference_every=3, use_graph=False)
+    e = gp._engine()
+    e.set_prefilter(prefilter)
+    try:
+        rg = gp.solve_sequence(frames, coords, mask, reference_every=3, use_graph=True)
+        re = gp.solve_sequence(frames, coords, mask, reference_every=3, use_graph=False)
+    finally:
+        e.set_prefilter(_native.PREFILTER_FIR)
     assert len(rg) == len(re) == len(frames) - 1
     for k, (a, b) in enumerate(zip(rg, re)):
         assert np.array_equal(a["p"], b["p"]), f"frame {k + 1}"
