@@ -98,12 +98,29 @@ double percentile(const std::vector<double>& s, double p) {
     return s[lo] + (pos - (double)lo) * (s[hi] - s[lo]);
 }
 
-// lower and upper quartiles of the finite values
+// lower and upper quartiles of the finite values: percentile() of the
+// sorted values, found by selection (the two order statistics each
+// percentile interpolates between) instead of a full sort; v is permuted
 void quartiles(std::vector<double>& v, double& lq, double& uq) {
     v.erase(std::remove_if(v.begin(), v.end(), [](double x) { return !std::isfinite(x); }), v.end());
-    std::sort(v.begin(), v.end());
-    lq = percentile(v, 25.0);
-    uq = percentile(v, 75.0);
+    if (v.empty()) {
+        lq = uq = 0.0;
+        return;
+    }
+    const size_t n = v.size();
+    auto pct = [&](double p) {
+        const double pos = p / 100.0 * ((double)n - 1.0);
+        const size_t lo = (size_t)std::floor(pos);
+        const size_t hi = std::min(lo + 1, n - 1);
+        std::nth_element(v.begin(), v.begin() + lo, v.end());
+        const double a = v[lo];
+        // after the selection every later element is >= v[lo]: the next
+        // order statistic is their minimum
+        const double b = hi == lo ? a : *std::min_element(v.begin() + lo + 1, v.end());
+        return a + (pos - (double)lo) * (b - a);
+    };
+    lq = pct(25.0);
+    uq = pct(75.0);
 }
 
 // f64::total_cmp order: by sign-folded bit pattern
@@ -432,6 +449,7 @@ struct Run {
     std::vector<double> p0;
     std::vector<dic_subset_result> ra, rb;
     std::vector<int64_t> retry;
+    std::vector<int32_t> cand_pos;  // node -> its index among the current candidates, else -1
     Workers workers;
 
     std::vector<int64_t> trusted(int64_t i, const uint8_t* skip) const {
@@ -552,17 +570,20 @@ struct Run {
         std::vector<int32_t> colour(cand.size(), -1);
         int32_t n_col = 0;
         std::vector<uint8_t> used;
+        if (cand_pos.size() != (size_t)t->n_nodes) cand_pos.assign(t->n_nodes, -1);
+        for (size_t k = 0; k < cand.size(); ++k) cand_pos[cand[k]] = (int32_t)k;
         for (size_t k = 0; k < cand.size(); ++k) {
             used.assign(n_col + 1, 0);
             for (const int64_t* q = t->full.begin(cand[k]); q != t->full.end(cand[k]); ++q) {
-                auto it = std::lower_bound(cand.begin(), cand.end(), *q);
-                if (it != cand.end() && *it == *q && colour[it - cand.begin()] >= 0) used[colour[it - cand.begin()]] = 1;
+                const int32_t j = cand_pos[*q];
+                if (j >= 0 && colour[j] >= 0) used[colour[j]] = 1;
             }
             int32_t c = 0;
             while (used[c]) ++c;
             colour[k] = c;
             n_col = std::max(n_col, c + 1);
         }
+        for (size_t k = 0; k < cand.size(); ++k) cand_pos[cand[k]] = -1;
         std::vector<int64_t> members;
         std::vector<double> pa;
         std::vector<dic_subset_result> kept;
