@@ -172,14 +172,6 @@ __device__ __forceinline__ void dbasis_f32(float t, float d[6]) {
     d[5] = 5.f * (float)GQ55 * (t2 * t2);
 }
 
-// Wavefront (64-lane) butterfly sum; every lane receives the same total and
-// the tree order is fixed, so results do not depend on what else runs.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // Value known to be equal on every lane, moved to scalar registers.
 __device__ __forceinline__ double uni(double v) {
     const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
@@ -285,18 +277,6 @@ __device__ __forceinline__ void wave_sums(const double* in, double* out) {
 #pragma unroll
         for (int j = 8 * G; j < V; ++j) out[j] = wave_sum_u(in[j]);
     }
-}
-
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
 }
 
 // A template offset pair as stored in LDS: two half-precision numbers

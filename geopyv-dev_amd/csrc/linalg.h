@@ -518,26 +518,4 @@ __device__ bool cholesky(const double* H, double* L) {
     return true;
 }
 
-// x = -(L Lᵀ)^-1 b
-template <int M>
-__device__ void cholesky_solve_neg(const double* L, const double* b, double* x) {
-    double y[M];
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        double s = b[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) s -= L[i * (i + 1) / 2 + k] * y[k];
-        y[i] = s / L[i * (i + 1) / 2 + i];
-    }
-#pragma unroll
-    for (int i = M - 1; i >= 0; --i) {
-        double s = y[i];
-#pragma unroll
-        for (int k = i + 1; k < M; ++k) s -= L[k * (k + 1) / 2 + i] * x[k];
-        x[i] = s / L[i * (i + 1) / 2 + i];
-    }
-#pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = -x[i];
-}
-
 }  // namespace geodic
