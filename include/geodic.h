@@ -184,9 +184,11 @@ int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_subset);
  * counterpart (the reference has no device schedule). */
 int dic_batch_autotune(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
 /* Solves only the listed subsets of a batch (each index at most once), each
- * from its row of p0 (count x 12, host; missing entries zero), and copies
- * their results to out (count records, host) in list order; the other
- * subsets keep their warps and results.  Waits for the device.  No history
+ * from its row of p0 (count x 12, host; missing entries zero), and writes
+ * their results to out (count records, host) in list order.  The batch's
+ * stored warps of the listed subsets become the solved ones; its stored
+ * result records (dic_batch_results) are left as they were, for listed and
+ * other subsets alike.  Waits for the device.  No history
  * (prm->want_history must be 0).  This is one round of node solves of the
  * reference's mesh: the per-node Subset::new + solve_icgn_result calls of
  * a layer-RG colour (mesh.rs:1224-1233) or of one serial RG step

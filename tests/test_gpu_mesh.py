@@ -102,6 +102,10 @@ def test_list_solve_equals_whole_batch(speckle):
                             max_norm=1e-5)
         for k in ("p", "c_zncc", "iterations", "converged", "status"):
             np.testing.assert_array_equal(part[k], whole[idx][k])
+        # a list solve writes its records only to its own output
+        after = b.results()
+        for k in ("p", "c_zncc", "iterations", "converged", "status"):
+            np.testing.assert_array_equal(after[k], whole[k])
     with pytest.raises(_native.GeodicError):
         b.solve_list(c["f"], c["g"], [1, 1], np.zeros((2, 12)))
     with pytest.raises(_native.GeodicError):
