@@ -10,15 +10,22 @@ using batch::BatchArgs;
 using batch::icgn_batch_kernel;
 using batch::kRedSlots;
 
+// P < 12 (a few pixels per thread) serves lists only.
 template <int M, int P, int WPS, int S>
 int launch_mpw(const BatchArgs& A, int blocks, size_t lds, hipStream_t stream) {
-    const void* k = A.res_list ? (const void*)icgn_batch_kernel<M, P, WPS, S, true>
-                               : (const void*)icgn_batch_kernel<M, P, WPS, S, false>;
-    DIC_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (A.res_list)
+    if constexpr (P < 12) {
+        const void* k = (const void*)icgn_batch_kernel<M, P, WPS, S, true>;
+        DIC_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         icgn_batch_kernel<M, P, WPS, S, true><<<blocks, 64 * WPS, lds, stream>>>(A);
-    else
-        icgn_batch_kernel<M, P, WPS, S, false><<<blocks, 64 * WPS, lds, stream>>>(A);
+    } else {
+        const void* k = A.res_list ? (const void*)icgn_batch_kernel<M, P, WPS, S, true>
+                                   : (const void*)icgn_batch_kernel<M, P, WPS, S, false>;
+        DIC_HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (A.res_list)
+            icgn_batch_kernel<M, P, WPS, S, true><<<blocks, 64 * WPS, lds, stream>>>(A);
+        else
+            icgn_batch_kernel<M, P, WPS, S, false><<<blocks, 64 * WPS, lds, stream>>>(A);
+    }
     DIC_HIP_TRY(hipGetLastError());
     return DIC_OK;
 }
@@ -33,7 +40,11 @@ int launch_ms(const BatchArgs& A, int blocks, size_t lds, bool s72, hipStream_t 
 }
 
 template <int M>
-int launch_m(const BatchArgs& A, int wps, int blocks, size_t lds, bool s72, hipStream_t stream) {
+int launch_m(const BatchArgs& A, int wps, int blocks, size_t lds, bool s72, hipStream_t stream, bool few) {
+    // a list spread over eight wavefronts per subset with at most four
+    // pixels per thread: the unrolled instantiation
+    if (few && wps == 8)
+        return s72 ? launch_mpw<M, 4, 8, 72>(A, blocks, lds, stream) : launch_mpw<M, 4, 8, 0>(A, blocks, lds, stream);
     switch (wps) {
         case 1: return launch_ms<M, 1>(A, blocks, lds, s72, stream);
         case 2: return launch_ms<M, 2>(A, blocks, lds, s72, stream);
@@ -140,9 +151,10 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
     const bool s72 = row <= 72;
     Ac.win_cap = (s72 ? 72 : row) * side;
     const size_t lds = sizeof(float) * (size_t)Ac.win_cap + 4 * (size_t)c.max_n;
-    if (b->max_order == 0) return launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st);
-    if (b->max_order == 1) return launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st);
-    return launch_m<12>(Ac, c.wps, (int)c.count, lds, s72, st);
+    const bool few = A.res_list && c.max_n <= 64 * 4 * c.wps;
+    if (b->max_order == 0) return launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st, few);
+    if (b->max_order == 1) return launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st, few);
+    return launch_m<12>(Ac, c.wps, (int)c.count, lds, s72, st, few);
 }
 
 }  // namespace

@@ -402,9 +402,24 @@ __device__ __forceinline__ float eval32(const float* win, const float* __restric
     return contract6(blk, (int)pitch, a, b) - shift;
 }
 
+// The wavefront partials of V values (rows of sRed, in wave order) summed
+// in wave order by lane j for value j, then broadcast from those lanes: WPS
+// LDS reads per lane instead of WPS x V, the same additions in the same
+// order as summing every value in every lane.
+template <int WPS, int V>
+__device__ __forceinline__ void combine_lanes(const double* sRed, int lane, double* v) {
+    const int j = lane < V ? lane : V - 1;
+    double t = sRed[j];
+#pragma unroll
+    for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
+#pragma unroll
+    for (int k = 0; k < V; ++k) v[k] = readlane_f64(t, k);
+}
+
 // Sums V FP64 values over the workgroup: transposed DPP/permlane trees per
-// wavefront, then the wavefront partials in wave order through LDS.  Result
-// uniform (SGPRs) in every wave.
+// wavefront, then the wavefront partials in wave order through LDS (with
+// four or more wavefronts, one value per lane).  Result uniform (SGPRs) in
+// every wave.
 template <int WPS, int V>
 __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int wave) {
     static_assert(V <= kRedSlots, "reduction buffer");
@@ -416,12 +431,16 @@ __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int
             for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
         }
         __syncthreads();
+        if constexpr (WPS >= 4) {
+            combine_lanes<WPS, V>(sRed, lane, v);
+        } else {
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-            double t = sRed[j];
+            for (int j = 0; j < V; ++j) {
+                double t = sRed[j];
 #pragma unroll
-            for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
-            v[j] = uni(t);
+                for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
+                v[j] = uni(t);
+            }
         }
         __syncthreads();
     } else {
@@ -437,7 +456,14 @@ __device__ __forceinline__ bool block_sum_lead(double* v, double* sRed, int lane
     static_assert(V <= kRedSlots, "reduction buffer");
     double w[V];
     wave_sums<V>(v, w);
-    if constexpr (WPS > 1) {
+    if constexpr (WPS >= 4) {
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
+        }
+        __syncthreads();
+        if (wave == 0) combine_lanes<WPS, V>(sRed, lane, v);
+    } else if constexpr (WPS > 1) {
         if (lane == 0 && wave > 0) {
 #pragma unroll
             for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
@@ -495,10 +521,11 @@ __device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr,
                                              int lane, int wave, double* sH, double* sRed) {
     constexpr int NT = 64 * WPS;
     constexpr int NE = (A1 - A0) * M - ((A1 - 1) * A1 - (A0 - 1) * A0) / 2;
+    constexpr int PU = P <= 4 ? P : 1;  // a few pixels per thread: overlap them
     float acc[NE > 0 ? NE : 1];
 #pragma unroll
     for (int q = 0; q < NE; ++q) acc[q] = 0.f;
-#pragma unroll 1
+#pragma unroll PU
     for (int k = 0; k < P; ++k) {
         const int i = tid + NT * k;
         if (i < n) {
@@ -806,8 +833,12 @@ __device__ __forceinline__ Plan load_plan(const Plan* sp) {
 // the result records to A.res_list in dispatch order (a separate
 // instantiation, so whole-batch solves compile exactly as without it)
 template <int M, int P, int WPS, int S, bool LIST = false>
-__global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_batch_kernel(BatchArgs A) {
+__global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATCH_OCC))) icgn_batch_kernel(BatchArgs A) {
     constexpr int NT = 64 * WPS;
+    // With a few pixels per thread (short lists spread over many wavefronts
+    // for latency) the pixel loops are unrolled so that the pixels' LDS
+    // reads and arithmetic overlap; otherwise they stay rolled.
+    constexpr int PU = P <= 4 ? P : 1;
     constexpr int NL = M * (M + 1) / 2;
     constexpr int K = Modes<M>::K;
     constexpr int NA = 3 + M;                          // per-iteration sums
@@ -841,6 +872,14 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
     // skips the atomic.
     const int64_t idx = blockIdx.x;
     if (idx >= A.n_sub) return;
+    // a short list's warps sit in host memory beside its dispatch order:
+    // the lead wavefront issues its row's read now, with the order's, so
+    // that the first plan does not wait for a second round trip
+    constexpr bool kEarlyP0 = LIST && P <= 4;
+    double p0v = 0.0;
+    if constexpr (kEarlyP0) {
+        if (wave == 0 && lane < M) p0v = A.p0_list[12 * idx + lane];
+    }
     const int64_t s = A.perm[idx];
     const int t = A.tpl_index[s];
     const int64_t st = A.tpl_start[t];
@@ -919,7 +958,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         const int LS = S > 0 ? S : W.ls;
         const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
         const float wwf = (float)LS;
-#pragma unroll 1
+#pragma unroll PU
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             float v = 0.f, gx = 0.f, gy = 0.f;
@@ -1015,7 +1054,12 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         if (wave == 0) {
             double p0[M];
 #pragma unroll
-            for (int k = 0; k < M; ++k) p0[k] = uni(LIST ? A.p0_list[12 * idx + k] : A.p0[12 * s + k]);
+            for (int k = 0; k < M; ++k) {
+                if constexpr (kEarlyP0)
+                    p0[k] = readlane_f64(p0v, k);
+                else
+                    p0[k] = uni(LIST ? A.p0_list[12 * idx + k] : A.p0[12 * s + k]);
+            }
             const Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
             if (lane == 0) {
                 sPlan = pl;
@@ -1031,7 +1075,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         float v2[V2];
 #pragma unroll
         for (int j = 0; j < V2; ++j) v2[j] = 0.f;
-#pragma unroll 1
+#pragma unroll PU
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
@@ -1125,7 +1169,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
             const int LS = S > 0 ? S : W.ls;
             const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
             const float wwf = (float)LS;
-#pragma unroll 1
+#pragma unroll PU
             for (int k = 0; k < P; ++k) {
                 const int i = tid + NT * k;
                 if (i < n) {
@@ -1143,7 +1187,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
                 }
             }
         } else {
-#pragma unroll 1
+#pragma unroll PU
             for (int k = 0; k < P; ++k) {
                 const int i = tid + NT * k;
                 if (i < n) {
@@ -1292,7 +1336,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : GEODIC_BATCH_OCC)) icgn_b
         const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
         const int LS = S > 0 ? S : W.ls;
         const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
-#pragma unroll 1
+#pragma unroll PU
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {

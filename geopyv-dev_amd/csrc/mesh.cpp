@@ -89,18 +89,10 @@ void build_topology(Topology& t) {
     if (t.order == 1) t.edge = t.full;
 }
 
-// percentile (mesh.rs:1919-1930) of an ascending array
-double percentile(const std::vector<double>& s, double p) {
-    if (s.empty()) return 0.0;
-    const double pos = p / 100.0 * ((double)s.size() - 1.0);
-    const size_t lo = (size_t)std::floor(pos);
-    const size_t hi = std::min(lo + 1, s.size() - 1);
-    return s[lo] + (pos - (double)lo) * (s[hi] - s[lo]);
-}
-
-// lower and upper quartiles of the finite values: percentile() of the
-// sorted values, found by selection (the two order statistics each
-// percentile interpolates between) instead of a full sort; v is permuted
+// lower and upper quartiles of the finite values: the percentile of
+// mesh.rs:1919-1930 (linear interpolation between the order statistics at
+// floor and floor + 1 of p/100 (n - 1)), found by selection instead of a
+// full sort; v is permuted
 void quartiles(std::vector<double>& v, double& lq, double& uq) {
     v.erase(std::remove_if(v.begin(), v.end(), [](double x) { return !std::isfinite(x); }), v.end());
     if (v.empty()) {

@@ -80,19 +80,23 @@ def _compare(ro, rg, disp_tol=1e-4, strain_tol=1e-6, zncc_tol=1e-6, iter_frac=0.
     np.testing.assert_allclose(rg["warps"][:, :6], ro["warps"][:, :6], rtol=0, atol=10 * disp_tol)
 
 
-def test_list_solve_equals_whole_batch(speckle):
+@pytest.mark.parametrize("order,wps", [(1, 2), (1, 8), (2, 8)])
+def test_list_solve_equals_whole_batch(speckle, order, wps):
     c = speckle
     xy, start = circle_template(c["radius"])
     n = len(c["nodes"])
-    b = _native.Batch(_native.Engine.get(0), c["nodes"], 1, np.zeros(n, np.int32), start,
+    b = _native.Batch(_native.Engine.get(0), c["nodes"], order, np.zeros(n, np.int32), start,
                       np.rint(xy).astype(np.int16))
     rng = np.random.default_rng(5)
     p0 = np.zeros((n, 12))
     p0[:, :2] = rng.uniform(-1, 1, size=(n, 2))
     # an explicit schedule: a short list otherwise takes more wavefronts per
     # subset than the whole batch, which rounds the FP32 engine's sums
-    # differently
-    b.set_schedule(3, 2)
+    # differently.  With eight wavefronts the list (r = 20: at most four
+    # pixels per thread) runs the unrolled list kernel, the whole batch the
+    # rolled one: their sums must agree bit for bit.
+    print(f"list vs whole batch: order {order}, {wps} wavefronts per subset", flush=True)
+    b.set_schedule(3, wps)
     for prec in (_native.PREC_PLAIN_F64, _native.PREC_COMP_F32):
         b.set_p0(p0)
         b.solve(c["f"], c["g"], precision=prec, max_iterations=50, tolerance=0.5, max_norm=1e-5)

@@ -3,6 +3,7 @@ emulation (tests/simt), under AddressSanitizer and UBSan, against the FP64
 oracle.  Catches out-of-bounds indexing, cross-lane operations reached by
 part of a wavefront, and logic errors without a GPU."""
 
+import fcntl
 import os
 import subprocess
 
@@ -13,7 +14,11 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "simt")
 
 @pytest.fixture(scope="module")
 def driver():
-    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    # one build at a time: parallel test workers share the folder, and a
+    # worker must not run a driver another is still writing
+    with open(os.path.join(HERE, ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-C", HERE, "-s"], check=True)
     return os.path.join(HERE, "simt_driver")
 
 
