@@ -43,6 +43,9 @@ int main(int argc, char** argv) {
     const int max_it = argc > 4 ? atoi(argv[4]) : 50;
     const int slack = argc > 5 ? atoi(argv[5]) : 3;
     const int wps = argc > 6 ? atoi(argv[6]) : 1;
+    // list mode: warps read from and results written to list buffers in
+    // dispatch order; at eight wavefronts the four-pixel list kernel
+    const int list = argc > 7 ? atoi(argv[7]) : 0;
     const int H = 96, W = 96, B = 20;
     const int HP = H + 2 * B, WP = W + 2 * B;
     uint64_t st = 12345;
@@ -129,6 +132,12 @@ int main(int argc, char** argv) {
     A.p0 = p0.data();
     A.p_out = pout.data();
     A.res = res.data();
+    std::vector<double> p0l(p0);  // dispatch order = subset order here
+    std::vector<dic_subset_result> resl(n);
+    if (list) {
+        A.p0_list = p0l.data();
+        A.res_list = resl.data();
+    }
     A.hist = hist.data();
     A.hist_cap = max_it;
     A.max_it = max_it;
@@ -151,6 +160,8 @@ int main(int argc, char** argv) {
                 simt::launch(n, 128, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 2, SS>(A); });
             else if (wps == 4)
                 simt::launch(n, 256, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 12, 4, SS>(A); });
+            else if (wps == 8 && list && npx <= 4 * 512)
+                simt::launch(n, 512, lds, [&] { geodic::batch::icgn_batch_kernel<MM, 4, 8, SS, true>(A); });
             else
                 std::abort();
         };
@@ -171,6 +182,7 @@ int main(int argc, char** argv) {
     std::vector<OrcResult> orc(n);
     orc_batch_solve(Cf.data(), Cg.data(), H, W, B, n, coords.data(), tpl_index.data(), xy.data(), tpl_start.data(),
                     orders.data(), p0.data(), 0, 0.75, 1e-5, max_it, orc.data(), 1);
+    if (list) res = resl;
     int bad = 0;
     double dumax = 0.0, dsmax = 0.0;
     for (int s = 0; s < n; ++s) {

@@ -22,12 +22,15 @@ def driver():
     return os.path.join(HERE, "simt_driver")
 
 
-@pytest.mark.parametrize("order,radius,nside,slack,wps", [(0, 10, 2, 3, 1), (1, 10, 2, 3, 1), (2, 10, 2, 3, 1),
-                                                          (1, 12, 1, 0, 2), (1, 7, 2, 40, 1), (2, 12, 1, 3, 4)])
-def test_batch_kernel_emulated_matches_oracle(driver, order, radius, nside, slack, wps):
+# the last two run the list kernel with four pixels per thread at eight
+# wavefronts per subset
+@pytest.mark.parametrize("order,radius,nside,slack,wps,list_mode",
+                         [(0, 10, 2, 3, 1, 0), (1, 10, 2, 3, 1, 0), (2, 10, 2, 3, 1, 0), (1, 12, 1, 0, 2, 0),
+                          (1, 7, 2, 40, 1, 0), (2, 12, 1, 3, 4, 0), (1, 12, 2, 3, 8, 1), (2, 10, 1, 3, 8, 1)])
+def test_batch_kernel_emulated_matches_oracle(driver, order, radius, nside, slack, wps, list_mode):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([driver, str(order), str(radius), str(nside), "50", str(slack), str(wps)], capture_output=True,
-                       text=True, timeout=600, env=env)
+    r = subprocess.run([driver, str(order), str(radius), str(nside), "50", str(slack), str(wps), str(list_mode)],
+                       capture_output=True, text=True, timeout=600, env=env)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     assert "mismatched 0" in r.stdout
