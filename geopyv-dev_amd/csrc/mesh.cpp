@@ -444,8 +444,10 @@ struct Run {
     std::vector<int32_t> cand_pos;  // node -> its index among the current candidates, else -1
     Workers workers;
 
-    std::vector<int64_t> trusted(int64_t i, const uint8_t* skip) const {
-        std::vector<int64_t> out;
+    // the node's trusted neighbours, in a buffer reused per host thread
+    const std::vector<int64_t>& trusted(int64_t i, const uint8_t* skip) const {
+        thread_local std::vector<int64_t> out;
+        out.clear();
         for (const int64_t* q = t->full.begin(i); q != t->full.end(i); ++q)
             if (qok[*q] && (skip ? !skip[*q] : stored[*q] != 0)) out.push_back(*q);
         return out;
@@ -552,17 +554,21 @@ struct Run {
     // nodes, lowest free colour); one colour at a time, every node from the
     // trusted set frozen before the colour, folded in ascending order.
     int expand(const std::vector<int64_t>& roots) {
+        // each unsolved edge neighbour once (marked in cand_pos while
+        // gathering), then in ascending order
+        if (cand_pos.size() != (size_t)t->n_nodes) cand_pos.assign(t->n_nodes, -1);
         std::vector<int64_t> cand;
         for (int64_t r : roots)
             for (const int64_t* q = t->edge.begin(r); q != t->edge.end(r); ++q)
-                if (!stored[*q]) cand.push_back(*q);
+                if (!stored[*q] && cand_pos[*q] == -1) {
+                    cand_pos[*q] = -2;
+                    cand.push_back(*q);
+                }
         std::sort(cand.begin(), cand.end());
-        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
         if (cand.empty()) return DIC_OK;
         std::vector<int32_t> colour(cand.size(), -1);
         int32_t n_col = 0;
         std::vector<uint8_t> used;
-        if (cand_pos.size() != (size_t)t->n_nodes) cand_pos.assign(t->n_nodes, -1);
         for (size_t k = 0; k < cand.size(); ++k) cand_pos[cand[k]] = (int32_t)k;
         for (size_t k = 0; k < cand.size(); ++k) {
             used.assign(n_col + 1, 0);
