@@ -10,7 +10,8 @@ nodes or more (90 % for second-order subsets): a node whose last update
 norm lies within round-off of max_norm may stop one iteration earlier or
 later, which moves it (and its neighbours' initial warps) by ~1e-6 px; the
 FP32 engine the same displacement bar.  A list solve returns exactly what the whole batch returns for the
-listed subsets under the same schedule.
+listed subsets under the same schedule, also over mixed template sizes (dispatched largest first, returned in
+list order); under the automatic schedule short lists agree with the whole batch to the FP32 engine's bar.
 """
 
 import numpy as np
@@ -115,6 +116,66 @@ def test_list_solve_equals_whole_batch(speckle, order, wps):
         b.solve_list(c["f"], c["g"], [1, 1], np.zeros((2, 12)))
     with pytest.raises(_native.GeodicError):
         b.solve_list(c["f"], c["g"], [n], np.zeros((1, 12)))
+
+
+def _two_radius_batch(c):
+    """The speckle nodes with circle templates r = 20 and r = 30 alternating."""
+    xy20, _ = circle_template(20)
+    xy30, _ = circle_template(30)
+    xy = np.vstack([xy20, xy30])
+    start = np.array([0, len(xy20), len(xy20) + len(xy30)], dtype=np.int64)
+    n = len(c["nodes"])
+    tpl = (np.arange(n) % 2).astype(np.int32)
+    b = _native.Batch(_native.Engine.get(0), c["nodes"], 1, tpl, start, np.rint(xy).astype(np.int16))
+    return b, n
+
+
+@pytest.mark.parametrize("wps", [4, 8])
+def test_mixed_size_list_equals_whole_batch(speckle, wps):
+    # a list over two template sizes is dispatched largest first and its
+    # results are returned in list order
+    c = speckle
+    b, n = _two_radius_batch(c)
+    rng = np.random.default_rng(7)
+    p0 = np.zeros((n, 12))
+    p0[:, :2] = rng.uniform(-1, 1, size=(n, 2))
+    print(f"mixed-size list vs whole batch: {wps} wavefronts per subset", flush=True)
+    b.set_schedule(3, wps)
+    for prec in (_native.PREC_PLAIN_F64, _native.PREC_COMP_F32):
+        b.set_p0(p0)
+        b.solve(c["f"], c["g"], precision=prec, max_iterations=50, tolerance=0.5, max_norm=1e-5)
+        whole = b.results()
+        idx = rng.permutation(n)[: n // 4]
+        part = b.solve_list(c["f"], c["g"], idx, p0[idx], precision=prec, max_iterations=50, tolerance=0.5,
+                            max_norm=1e-5)
+        np.testing.assert_array_equal(part["n_px"], whole[idx]["n_px"])
+        for k in ("p", "c_zncc", "eta_u", "iterations", "converged", "status"):
+            np.testing.assert_array_equal(part[k], whole[idx][k])
+
+
+def test_mixed_size_short_lists_in_latency_mode(speckle):
+    # automatic schedule: a short list spreads each size class over more
+    # wavefronts (r = 20 through the four-pixel list kernel, r = 30 through
+    # the rolled one), which rounds the FP32 sums differently from the whole
+    # batch; results agree to the engine's bar
+    c = speckle
+    b, n = _two_radius_batch(c)
+    rng = np.random.default_rng(8)
+    p0 = np.zeros((n, 12))
+    p0[:, :2] = rng.uniform(-1, 1, size=(n, 2))
+    b.set_p0(p0)
+    b.solve(c["f"], c["g"], precision=_native.PREC_COMP_F32, max_iterations=50, tolerance=0.5, max_norm=1e-5)
+    whole = b.results()
+    for size in (1, 7, 60):
+        idx = rng.permutation(n)[:size]
+        print(f"latency-mode list of {size}", flush=True)
+        part = b.solve_list(c["f"], c["g"], idx, p0[idx], precision=_native.PREC_COMP_F32, max_iterations=50,
+                            tolerance=0.5, max_norm=1e-5)
+        np.testing.assert_array_equal(part["n_px"], whole[idx]["n_px"])
+        np.testing.assert_array_equal(part["status"], whole[idx]["status"])
+        ok = whole[idx]["converged"] == 1
+        assert np.abs(part["p"][ok, :2] - whole[idx]["p"][ok, :2]).max() < 1e-4
+        assert np.mean(part["converged"] == whole[idx]["converged"]) >= 0.95
 
 
 @pytest.mark.parametrize("pre", ["RG", "layer-RG"])
