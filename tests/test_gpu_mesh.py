@@ -81,7 +81,7 @@ def _compare(ro, rg, disp_tol=1e-4, strain_tol=1e-6, zncc_tol=1e-6, iter_frac=0.
     np.testing.assert_allclose(rg["warps"][:, :6], ro["warps"][:, :6], rtol=0, atol=10 * disp_tol)
 
 
-@pytest.mark.parametrize("order,wps", [(1, 2), (1, 8), (2, 8)])
+@pytest.mark.parametrize("order,wps", [(1, 2), (0, 8), (1, 8), (2, 8)])
 def test_list_solve_equals_whole_batch(speckle, order, wps):
     c = speckle
     xy, start = circle_template(c["radius"])
