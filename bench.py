@@ -312,6 +312,59 @@ def run_mesh(args, torch, gp, _native, workloads):
     print(json.dumps(out))
 
 
+def run_lists(args, torch, gp, _native, workloads):
+    """List mode: short list solves (dic_batch_solve_list) on the headline
+    pair, as a mesh traversal issues them (mesh.rs:1224-1233 solves one
+    colour's nodes at a time): 1, 16, 100 and 400 subsets drawn at random
+    from the 40 000, each from its rounded-truth warp, timed one list at a
+    time with the images resident; a list's wall time includes its launch,
+    the device work and the wait for it."""
+    w = workloads.uniform_mesh_pair(seed=1)
+    ref, tar, coords = w["ref"], w["tar"], w["coords"]
+    n = len(coords)
+    mask = gp.Mask(mask_type="local", shape="circle", size=w["radius"])
+    xy = np.rint(mask.coords).astype(np.int16)
+    eng = gp._engine()
+    dref = torch.from_numpy(ref).cuda()
+    dtar = torch.from_numpy(tar).cuda()
+    torch.cuda.synchronize()
+    f_img = _native.DeviceImage(eng, device_ptr=dref.data_ptr(), shape=ref.shape, pix_type=_native.PIX_U8, border=20)
+    g_img = _native.DeviceImage(eng, device_ptr=dtar.data_ptr(), shape=ref.shape, pix_type=_native.PIX_U8, border=20)
+    b = _native.Batch(eng, coords, 1, np.zeros(n, np.int32), np.array([0, len(xy)], np.int64), xy)
+    rng = np.random.default_rng(0)
+    reps = max(args.steps, 5)
+    sizes = {}
+    for size in (1, 16, 100, 400):
+        idx = np.sort(rng.choice(n, size, replace=False)).astype(np.int32)
+        p0 = w["p0"][idx]
+        kw = dict(precision=_native.PREC_COMP_F32, max_iterations=args.max_iterations, tolerance=0.75,
+                  max_norm=1e-5)
+        for _ in range(max(args.warmup, 1)):
+            b.solve_list(f_img, g_img, idx, p0, **kw)
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = b.solve_list(f_img, g_img, idx, p0, **kw)
+            ts.append(time.perf_counter() - t)
+        us = np.array(ts) * 1e6
+        sizes[str(size)] = {"p50_us": round(float(np.median(us)), 1), "p99_us": round(float(np.percentile(us, 99)), 1),
+                            "subsets_per_s": round(size / (float(np.median(us)) * 1e-6), 1),
+                            "iterations_mean": round(float(r["iterations"].mean()), 2),
+                            "iterations_max": int(r["iterations"].max())}
+        log(f"list of {size}: p50 {sizes[str(size)]['p50_us']} us, iterations mean "
+            f"{sizes[str(size)]['iterations_mean']} max {sizes[str(size)]['iterations_max']}")
+    out = {
+        "metric": "list solve latency: wall time of one short list solve (FP32 engine), 100-subset list",
+        "value": sizes["100"]["p50_us"], "unit": "us", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+        "higher_is_better": False, "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "lists: subsets of the 2048x2048 uniform-mesh pair (40000 circle subsets r=20), "
+                               "lists of 1/16/100/400 drawn at random, rounded-truth initial warps, order 1",
+                   "max_norm": 1e-5},
+        "lists": sizes,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_serving(args, gp, workloads):
     """Serving loop: Poisson arrivals of correlation jobs, image sizes drawn
     from {1024², 2048², 4096x3000} (speckle density 0.012/px², σ = 2 px,
@@ -397,7 +450,7 @@ def main():
                     help="only run a 512 MiB device-to-device copy (known HBM bytes) for counter scaling")
     ap.add_argument("--engine", choices=["both", "f32"], default="both",
                     help="f32: time only the batched engine (for profiler runs)")
-    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band", "sequence", "serving", "mesh"],
+    ap.add_argument("--workload", choices=["uniform_mesh", "shear_band", "sequence", "serving", "mesh", "lists"],
                     default="uniform_mesh",
                     help="shear_band: uneven radii, second-order warp; sequence: incremental frames through a "
                          "captured frame graph (each its own line, not the headline)")
@@ -435,6 +488,9 @@ def main():
         return
     if args.workload == "mesh":
         run_mesh(args, torch, gp, _native, workloads)
+        return
+    if args.workload == "lists":
+        run_lists(args, torch, gp, _native, workloads)
         return
 
     t0 = time.time()
