@@ -38,6 +38,10 @@ compose_icgn           golden vector (subset.rs:1790-1810); identity, order
 convergence_norm       closed forms (subset.rs:1425-1456)
 apply_warp / sdi       every-slot quad warp (subset.rs:2221-2259)
 zn_slice / eta         hand-computed values (subset.rs:1460-1616)
+Subset::new statistics f_m, delta_f, SSSIG, sigma against NumPy on the
+(f_m, delta_f, sssig,  oracle's own f / grad_f; a hand-computed four-pixel
+sigma)                 case on a linear ramp (f_m 332.5, delta_f sqrt 13,
+                       sigma sqrt(13/4), SSSIG 26)
 subset_solve (ICGN /   synthetic translation recovered to 1e-4 px;
 FAGN)                  ZNSSD = 0 on identical images; quality_ok needs
                        convergence; max_iterations=1 not converged;

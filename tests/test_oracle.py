@@ -438,3 +438,48 @@ def test_featureless_subset_is_invalid_input():
     off, _ = O.mask_local("circle", 5)
     r = O.subset_solve(C, C, (64, 64), 20, [32.0, 32.0], off, 1)
     assert r["status"] == 3
+
+
+# ---------------------------------------------------------------------------
+# Subset::new statistics (subset.rs:885-899): f_m = mean f, delta_f =
+# sqrt(sum (f - f_m)^2), SSSIG = sum 1/2 (gx^2 + gy^2), sigma =
+# sqrt(sum (f - f_m)^2 / n), checked against NumPy on the oracle's own f and
+# grad_f (themselves pinned through eval / eval_grad above) and against a
+# hand-computed four-pixel case (the pattern of test_subset.py:49-53).
+# ---------------------------------------------------------------------------
+def test_subset_new_statistics_match_numpy(speckle_pair):
+    f, g, Cf, Cg = speckle_pair
+    for size, centre in ((15, [64.0, 64.0]), (9, [40.3, 71.8])):
+        off, _ = O.mask_local("circle", size)
+        r = O.subset_solve(Cf, Cg, f.shape, 20, centre, off, 1, max_iterations=0, want_arrays=True)
+        fv, gr = r["f"], r["grad_f"]
+        n = len(off)
+        assert r["n_px"] == n and len(fv) == n
+        f_m = np.mean(fv)
+        d = fv - f_m
+        assert abs(r["f_m"] - f_m) <= 1e-12 * abs(f_m)
+        assert abs(r["delta_f"] - np.sqrt(np.sum(d * d))) <= 1e-12 * r["delta_f"]
+        assert abs(r["sssig"] - np.sum(0.5 * (gr[:, 0] ** 2 + gr[:, 1] ** 2))) <= 1e-12 * r["sssig"]
+        assert abs(r["sigma"] - np.sqrt(np.sum(d * d) / n)) <= 1e-12 * r["sigma"]
+        # the f_coords are the offsets about the centre (subset.rs:866-871)
+        assert np.array_equal(r["f_coords"], off + np.asarray(centre))
+
+
+def test_subset_new_statistics_four_pixel_known_answer():
+    # I(x, y) = 2x + 3y + 10 on a 128^2 image: a B-spline interpolant
+    # reproduces the ramp far from the padded border, so at the four pixels
+    # about (64, 64) f = 330, 332, 333, 335 and grad f = (2, 3) exactly:
+    # f_m = 332.5, sum (f - f_m)^2 = 13, delta_f = sqrt(13), sigma =
+    # sqrt(13 / 4), SSSIG = 4 * 1/2 * (4 + 9) = 26.
+    yy, xx = np.mgrid[0:128, 0:128].astype(np.float64)
+    img = 2.0 * xx + 3.0 * yy + 10.0
+    C = O.get_c(img, 20)
+    off = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]])
+    r = O.subset_solve(C, C, img.shape, 20, [64.0, 64.0], off, 1, max_iterations=0, want_arrays=True)
+    assert r["status"] == 0 and r["n_px"] == 4
+    assert np.allclose(r["f"], [330.0, 332.0, 333.0, 335.0], rtol=0, atol=1e-9)
+    assert np.allclose(r["grad_f"], [[2.0, 3.0]] * 4, rtol=0, atol=1e-9)
+    assert abs(r["f_m"] - 332.5) < 1e-9
+    assert abs(r["delta_f"] - np.sqrt(13.0)) < 1e-9
+    assert abs(r["sigma"] - np.sqrt(13.0 / 4.0)) < 1e-9
+    assert abs(r["sssig"] - 26.0) < 1e-8
