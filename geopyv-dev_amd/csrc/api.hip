@@ -99,7 +99,7 @@ int dic_engine_create(int device, dic_engine** out) {
              hipEventCreateWithFlags(&e->ev_join[k], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         set_error("dic_engine_create: stream/event creation failed");
-        delete d;
+        dic_engine_destroy(d);
         return DIC_ERR_DEVICE;
     }
     *out = d;
@@ -110,15 +110,13 @@ void dic_engine_destroy(dic_engine* d) {
     if (!d) return;
     Engine* e = &d->e;
     (void)hipSetDevice(e->device);
-    (void)hipStreamSynchronize(e->stream);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto& pr : e->twiddles) (void)hipFree(pr.second);
     for (auto& pr : e->fir_taps) (void)hipFree(pr.second);
     if (e->pre_tmp) (void)hipFree(e->pre_tmp);
     if (e->pix_stage) (void)hipFree(e->pix_stage);
-    (void)hipEventDestroy(e->ev_pre0);
-    (void)hipEventDestroy(e->ev_pre1);
-    (void)hipEventDestroy(e->ev_sol0);
-    (void)hipEventDestroy(e->ev_sol1);
+    for (hipEvent_t ev : {e->ev_pre0, e->ev_pre1, e->ev_sol0, e->ev_sol1})
+        if (ev) (void)hipEventDestroy(ev);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     for (int k = 0; k < Engine::kAux; ++k) {
         if (e->aux[k]) {
@@ -127,7 +125,7 @@ void dic_engine_destroy(dic_engine* d) {
         }
         if (e->ev_join[k]) (void)hipEventDestroy(e->ev_join[k]);
     }
-    (void)hipStreamDestroy(e->stream);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
     delete d;
 }
 
@@ -169,7 +167,7 @@ int dic_engine_timing(dic_engine* d, double* prefilter_ms, double* solve_ms) {
     return DIC_OK;
 }
 
-int64_t dic_engine_bytes_in_use(dic_engine* d) { return d ? d->e.bytes_in_use : 0; }
+int64_t dic_engine_bytes_in_use(dic_engine* d) { return d ? d->e.bytes_in_use.load() : 0; }
 
 // ---------------------------------------------------------------------------
 // images
@@ -256,6 +254,7 @@ int dic_image_coef(const dic_image* di, double* out) {
     const Image* im = &di->im;
     Engine* e = im->eng;
     DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
     DIC_HIP_TRY(hipMemcpyAsync(out, im->coef64, sizeof(double) * im->hp * im->wp, hipMemcpyDeviceToHost, e->stream));
     DIC_HIP_TRY(hipStreamSynchronize(e->stream));
     return DIC_OK;
@@ -274,6 +273,7 @@ void dic_image_destroy(dic_image* di) {
     Image* im = &di->im;
     Engine* e = im->eng;
     (void)hipSetDevice(e->device);
+    std::lock_guard<std::mutex> lk(e->mu);
     (void)hipStreamSynchronize(e->stream);
     const size_t np = (size_t)im->hp * im->wp;
     dev_free(e, im->coef64, np * sizeof(double));
@@ -413,8 +413,12 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
         b->max_n = std::max<int32_t>(b->max_n, (int32_t)nt);
         b->max_side = std::max(b->max_side, std::max(x1 - x0 + 1, y1 - y0 + 1));
     }
+    // the plain kernel's per-pixel arrays: subset s's start at (pixels of
+    // the subsets before it) x (arrays per pixel)
+    b->h_scr_off.resize(n_subsets);
     for (int64_t s = 0; s < n_subsets; ++s) {
         const int t = tpl_index[s];
+        b->h_scr_off[s] = b->total_px;
         b->total_px += tpl_start[t + 1] - tpl_start[t];
     }
     b->h_coords.assign(coords, coords + 2 * n_subsets);
@@ -461,6 +465,7 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     H2D(b->d_tpl_xy_eng, packed_eng.data(), sizeof(int32_t) * n_off);
     H2D(b->d_tpl_ext, ext.data(), sizeof(int32_t) * 4 * n_templates);
     H2D(b->d_order_perm, perm.data(), sizeof(int32_t) * n_subsets);
+    H2D(b->d_scr_off, b->h_scr_off.data(), sizeof(int64_t) * n_subsets);
     if (!st && hipMemsetAsync(b->d_p0, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
     if (!st && hipMemsetAsync(b->d_p, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
     if (!st && hipMemsetAsync(b->d_res, 0, sizeof(dic_subset_result) * n_subsets, e->stream) != hipSuccess)
@@ -481,6 +486,7 @@ int dic_batch_set_p0(dic_batch* db, const double* p0, int mem) {
     Batch* b = &db->b;
     Engine* e = b->eng;
     DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
     DIC_HIP_TRY(hipMemcpyAsync(b->d_p0, p0, sizeof(double) * 12 * b->n,
                                mem == DIC_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, e->stream));
     if (mem == DIC_MEM_HOST) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -506,6 +512,7 @@ int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_subset) {
     CHECK_ARG(waves_per_subset == 0 || waves_per_subset == 1 || waves_per_subset == 2 || waves_per_subset == 4 ||
                   waves_per_subset == 8,
               "invalid input: waves_per_subset must be 0 (automatic), 1, 2, 4 or 8");
+    std::lock_guard<std::mutex> lk(db->b.eng->mu);
     db->b.win_slack = win_slack;
     db->b.waves_per_subset = waves_per_subset;
     return DIC_OK;
@@ -617,6 +624,7 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
     b->last_method = prm->method;
     b->last_max_it = prm->max_iterations;
     b->last_history = prm->want_history != 0;
+    b->arrays_valid = prm->precision == DIC_PREC_PLAIN_F64;
     return DIC_OK;
 }
 
@@ -643,12 +651,32 @@ int dic_batch_autotune(dic_batch* db, const dic_image* f, const dic_image* g, co
     return DIC_OK;
 }
 
+int dic_batch_get_schedule(dic_batch* db, int32_t* wps, int32_t* slack) {
+    CHECK_ARG(db && wps && slack, "dic_batch_get_schedule: NULL argument");
+    Batch* b = &db->b;
+    std::lock_guard<std::mutex> lk(b->eng->mu);
+    for (int k = 0; k < 4; ++k) {
+        const int base = 1 << k;
+        if (b->waves_per_subset > 0) {
+            wps[k] = b->waves_per_subset;
+            slack[k] = b->win_slack;
+        } else if (b->tuned_wps[base] > 0) {
+            wps[k] = b->tuned_wps[base];
+            slack[k] = b->tuned_slack[base];
+        } else {
+            wps[k] = base;
+            slack[k] = b->win_slack;
+        }
+    }
+    return DIC_OK;
+}
 
 int dic_batch_results(dic_batch* db, dic_subset_result* out) {
     CHECK_ARG(db && out, "dic_batch_results: NULL argument");
     Batch* b = &db->b;
     Engine* e = b->eng;
     DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
     DIC_HIP_TRY(hipMemcpyAsync(out, b->d_res, sizeof(dic_subset_result) * b->n, hipMemcpyDeviceToHost, e->stream));
     DIC_HIP_TRY(hipStreamSynchronize(e->stream));
     return DIC_OK;
@@ -657,12 +685,13 @@ int dic_batch_results(dic_batch* db, dic_subset_result* out) {
 int dic_batch_history(dic_batch* db, double* out) {
     CHECK_ARG(db && out, "dic_batch_history: NULL argument");
     Batch* b = &db->b;
+    Engine* e = b->eng;
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
     if (!b->last_history || !b->d_hist) {
         set_error("not solved: no history was recorded (want_history = 0)");
         return DIC_ERR_NOT_SOLVED;
     }
-    Engine* e = b->eng;
-    DIC_HIP_TRY(hipSetDevice(e->device));
     DIC_HIP_TRY(hipMemcpyAsync(out, b->d_hist, sizeof(double) * 4 * b->n * b->hist_cap, hipMemcpyDeviceToHost,
                                e->stream));
     DIC_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -673,16 +702,19 @@ int dic_batch_subset_arrays(dic_batch* db, int64_t s, double* f, double* grad_f)
     CHECK_ARG(db, "dic_batch_subset_arrays: NULL batch");
     Batch* b = &db->b;
     CHECK_ARG(s >= 0 && s < b->n, "invalid input: subset index out of range");
-    if (b->last_precision != DIC_PREC_PLAIN_F64 || b->h_scr_off.empty()) {
-        set_error("not solved: per-pixel arrays need a DIC_PREC_PLAIN_F64 solve");
-        return DIC_ERR_NOT_SOLVED;
-    }
     Engine* e = b->eng;
     DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!b->arrays_valid || !b->d_scratch) {
+        set_error("not solved: per-pixel arrays need a whole-batch DIC_PREC_PLAIN_F64 solve (dic_batch_solve) "
+                  "as the last solve of the batch");
+        return DIC_ERR_NOT_SOLVED;
+    }
     const int t = b->h_tpl_index[s];
     const int64_t n = b->h_tpl_start[t + 1] - b->h_tpl_start[t];
     std::vector<double> buf(3 * n);
-    DIC_HIP_TRY(hipMemcpyAsync(buf.data(), b->d_scratch + b->h_scr_off[s], sizeof(double) * 3 * n,
+    const int64_t at = (int64_t)plain_arrays(b->last_method) * b->h_scr_off[s];
+    DIC_HIP_TRY(hipMemcpyAsync(buf.data(), b->d_scratch + at, sizeof(double) * 3 * n,
                                hipMemcpyDeviceToHost, e->stream));
     DIC_HIP_TRY(hipStreamSynchronize(e->stream));
     if (f) std::memcpy(f, buf.data(), sizeof(double) * n);
@@ -704,8 +736,11 @@ void dic_batch_destroy(dic_batch* db) {
     if (!db) return;
     Batch* b = &db->b;
     (void)hipSetDevice(b->eng->device);
-    (void)hipStreamSynchronize(b->eng->stream);
-    batch_free(b);
+    {
+        std::lock_guard<std::mutex> lk(b->eng->mu);
+        (void)hipStreamSynchronize(b->eng->stream);
+        batch_free(b);
+    }
     delete db;
 }
 
@@ -876,8 +911,14 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
     if (prm->precision == DIC_PREC_PLAIN_F64) {
         std::memcpy(h_idx, idx, sizeof(int32_t) * count);
         std::memcpy(h_p0, p0, sizeof(double) * 12 * count);
+        // the listed subsets' per-pixel arrays overwrite those of the
+        // stored whole-batch results
+        b->arrays_valid = false;
         st = launch_icgn_plain(b, f, g, prm, d_idx, count, d_src, d_res);
-        if (st) return st;
+        if (st) {
+            (void)hipStreamSynchronize(e->stream);  // nothing may still write the staging
+            return st;
+        }
         DIC_HIP_TRY(hipStreamSynchronize(e->stream));
         std::memcpy(out, h_res, sizeof(dic_subset_result) * count);
     } else {
@@ -894,14 +935,15 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
             std::memcpy(h_p0 + 12 * q, p0 + 12 * (int64_t)pos[q], sizeof(double) * 12);
         }
         st = launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count, true, d_src, d_res);
-        if (st) return st;
+        if (st) {
+            (void)hipStreamSynchronize(e->stream);  // nothing may still write the staging
+            return st;
+        }
         DIC_HIP_TRY(hipStreamSynchronize(e->stream));
         for (int64_t q = 0; q < count; ++q) out[pos[q]] = h_res[q];
     }
-    b->last_precision = prm->precision;
-    b->last_method = prm->method;
-    b->last_max_it = prm->max_iterations;
-    b->last_history = false;
+    // the batch's stored result records, history and their solve settings
+    // are the last whole-batch solve's: a list leaves them as they were
     return DIC_OK;
 }
 

@@ -177,20 +177,26 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
         DIC_HIP_TRY(hipEventRecord(e->ev_fork, e->stream));
         for (int k = 0; k + 1 < nstreams; ++k) DIC_HIP_TRY(hipStreamWaitEvent(e->aux[k], e->ev_fork, 0));
     }
-    for (size_t k = 0; k < cls.size(); ++k) {
+    int rc = DIC_OK;
+    for (size_t k = 0; k < cls.size() && !rc; ++k) {
         hipStream_t st = (k % nstreams == 0) ? e->stream : e->aux[k % nstreams - 1];
-        const int r = launch_class(b, A, cls[k], st, d_perm);
-        if (r) return r;
+        rc = launch_class(b, A, cls[k], st, d_perm);
     }
+    // every forked stream is joined back, also after a failed launch, so
+    // that a wait on the engine's stream covers the classes already running
+    // (and a capture that started on it can end)
     for (int k = 0; k + 1 < nstreams; ++k) {
-        DIC_HIP_TRY(hipEventRecord(e->ev_join[k], e->aux[k]));
-        DIC_HIP_TRY(hipStreamWaitEvent(e->stream, e->ev_join[k], 0));
+        if (hipEventRecord(e->ev_join[k], e->aux[k]) != hipSuccess ||
+            hipStreamWaitEvent(e->stream, e->ev_join[k], 0) != hipSuccess) {
+            if (!rc) set_error("launch_icgn_batch: joining a size-class stream failed");
+            rc = rc ? rc : DIC_ERR_DEVICE;
+        }
     }
-    return DIC_OK;
+    return rc;
 }
 
 // Every class alone on the engine's stream, for each candidate (fewest
-// wavefronts or twice that, window slack 1 or 3 pixels), best of two
+// wavefronts or twice that, window slack 1 or 3 pixels), best of three
 // timed runs; the fastest is kept for later solves.  The results the
 // batch holds afterwards are those of the last candidate.
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
@@ -206,29 +212,36 @@ int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solv
     DIC_HIP_TRY(hipEventCreate(&t1));
     int rc = DIC_OK;
     for (const SizeClass& c0 : cls) {
+        // the untuned schedule (fewest wavefronts, the batch's slack) is
+        // timed first and kept unless a candidate beats it by more than 3 %,
+        // so that timing noise between near-equal candidates does not flip
+        // the choice from run to run
         float best = 1e30f;
         int bw = c0.base_wps, bs = b->win_slack;
-        for (int w = c0.base_wps; w <= std::min(8, 2 * c0.base_wps) && !rc; w *= 2)
-            for (int sl : {1, 3}) {
-                SizeClass c = c0;
-                c.wps = w;
-                c.slack = sl;
-                float tmin = 1e30f;
-                for (int rep = 0; rep < 2 && !rc; ++rep) {
-                    if (hipEventRecord(t0, e->stream) != hipSuccess) rc = DIC_ERR_DEVICE;
-                    if (!rc) rc = launch_class(b, A, c, e->stream);
-                    if (!rc && hipEventRecord(t1, e->stream) != hipSuccess) rc = DIC_ERR_DEVICE;
-                    float ms = 0.f;
-                    if (!rc && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
-                        rc = DIC_ERR_DEVICE;
-                    tmin = std::min(tmin, ms);
-                }
-                if (!rc && tmin < best) {
-                    best = tmin;
-                    bw = w;
-                    bs = sl;
-                }
+        std::vector<std::pair<int, int>> cand = {{c0.base_wps, b->win_slack}};
+        for (int w = c0.base_wps; w <= std::min(8, 2 * c0.base_wps); w *= 2)
+            for (int sl : {1, 3})
+                if (!(w == c0.base_wps && sl == b->win_slack)) cand.push_back({w, sl});
+        for (size_t ci = 0; ci < cand.size() && !rc; ++ci) {
+            SizeClass c = c0;
+            c.wps = cand[ci].first;
+            c.slack = cand[ci].second;
+            float tmin = 1e30f;
+            for (int rep = 0; rep < 3 && !rc; ++rep) {
+                if (hipEventRecord(t0, e->stream) != hipSuccess) rc = DIC_ERR_DEVICE;
+                if (!rc) rc = launch_class(b, A, c, e->stream);
+                if (!rc && hipEventRecord(t1, e->stream) != hipSuccess) rc = DIC_ERR_DEVICE;
+                float ms = 0.f;
+                if (!rc && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+                    rc = DIC_ERR_DEVICE;
+                tmin = std::min(tmin, ms);
             }
+            if (!rc && (ci == 0 || tmin < 0.97f * best)) {
+                best = tmin;
+                bw = c.wps;
+                bs = c.slack;
+            }
+        }
         if (rc) break;
         b->tuned_wps[c0.base_wps] = bw;
         b->tuned_slack[c0.base_wps] = bs;

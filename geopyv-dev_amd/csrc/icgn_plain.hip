@@ -31,8 +31,9 @@ struct PlainArgs {
     const int32_t* tpl_index;
     const int64_t* tpl_start;
     const int32_t* tpl_xy;
-    const int64_t* scr_off;
+    const int64_t* scr_off;  // pixels before each subset's arrays
     double* scratch;
+    int arrays;              // per-pixel doubles: 4 (ICGN) or 6 (FAGN)
     const double* p0;
     double* p_out;
     dic_subset_result* res;
@@ -201,7 +202,7 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
         if (lane == 0) R->status = DIC_ERR_INVALID_INPUT;
         return;
     }
-    double* f = A.scratch + A.scr_off[s];
+    double* f = A.scratch + (int64_t)A.arrays * A.scr_off[s];
     double* gxa = f + n;
     double* gya = gxa + n;
     double* g = gya + n;
@@ -543,34 +544,27 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
 }  // namespace
 
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* list,
-                      int64_t count, const double* p0_list, dic_subset_result* res_list) {
+                      int64_t count, const double* p0_list, dic_subset_result* res_list, double* ext_scratch,
+                      size_t ext_doubles) {
     Engine* e = b->eng;
-    const int fagn = prm->method == DIC_METHOD_FAGN;
-    const int arrays = fagn ? 6 : 4;
+    const int arrays = plain_arrays(prm->method);
     const size_t need = (size_t)b->total_px * arrays;
-    if (b->scratch_doubles < need) {
-        if (b->d_scratch) dev_free(e, b->d_scratch, b->scratch_doubles * sizeof(double));
-        b->d_scratch = nullptr;
-        b->scratch_doubles = 0;
-        int st = dev_alloc(e, (void**)&b->d_scratch, need * sizeof(double));
-        if (st) return st;
-        b->scratch_doubles = need;
-    }
-    // per-subset scratch offsets depend on the per-pixel array count; laid
-    // out again only when that changes
-    if (b->scr_arrays != arrays) {
-        std::vector<int64_t> off(b->n);
-        int64_t acc = 0;
-        for (int64_t s = 0; s < b->n; ++s) {
-            off[s] = acc;
-            const int t = b->h_tpl_index[s];
-            acc += (b->h_tpl_start[t + 1] - b->h_tpl_start[t]) * arrays;
+    double* scratch = ext_scratch;
+    if (scratch) {
+        if (need > ext_doubles) {
+            set_error("invalid input: the batch's per-pixel arrays exceed the scratch");
+            return DIC_ERR_INVALID_INPUT;
         }
-        DIC_HIP_TRY(hipMemcpyAsync(b->d_scr_off, off.data(), sizeof(int64_t) * b->n, hipMemcpyHostToDevice, e->stream));
-        // the host vector must outlive the asynchronous copy
-        DIC_HIP_TRY(hipStreamSynchronize(e->stream));
-        b->h_scr_off.swap(off);
-        b->scr_arrays = arrays;
+    } else {
+        if (b->scratch_doubles < need) {
+            if (b->d_scratch) dev_free(e, b->d_scratch, b->scratch_doubles * sizeof(double));
+            b->d_scratch = nullptr;
+            b->scratch_doubles = 0;
+            int st = dev_alloc(e, (void**)&b->d_scratch, need * sizeof(double));
+            if (st) return st;
+            b->scratch_doubles = need;
+        }
+        scratch = b->d_scratch;
     }
     PlainArgs A;
     A.fC = f->coef64;
@@ -587,7 +581,8 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.tpl_start = b->d_tpl_start;
     A.tpl_xy = b->d_tpl_xy;
     A.scr_off = b->d_scr_off;
-    A.scratch = b->d_scratch;
+    A.scratch = scratch;
+    A.arrays = arrays;
     A.p0 = b->d_p0;
     A.p0_list = list ? p0_list : nullptr;
     A.res_list = list ? res_list : nullptr;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
@@ -33,7 +34,7 @@ struct Engine {
     hipStream_t aux[kAux] = {nullptr, nullptr, nullptr};  // size classes of a batch run here too
     hipEvent_t ev_fork = nullptr, ev_join[kAux] = {nullptr, nullptr, nullptr};
     bool pre_recorded = false, sol_recorded = false;
-    int64_t bytes_in_use = 0;
+    std::atomic<int64_t> bytes_in_use{0};  // updated by allocations and frees of any thread
     int num_cus = 256;
     bool capturing = false;  // recording a graph: no timing events, no host synchronisation
     std::mutex mu;
@@ -96,7 +97,7 @@ struct Batch {
     int32_t waves_per_subset = 0;   // batched engine: wavefronts per subset (0 = smallest that fits)
     int32_t tuned_wps[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};    // autotuned wavefronts by a class's fewest (0 = none)
     int32_t tuned_slack[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t* d_scr_off = nullptr;   // n: offset of each subset's scratch
+    int64_t* d_scr_off = nullptr;   // n: h_scr_off on the device
     double* d_p0 = nullptr;         // n x 12
     double* d_p = nullptr;          // n x 12 final warps
     dic_subset_result* d_res = nullptr;
@@ -112,8 +113,9 @@ struct Batch {
     // host copies
     std::vector<int64_t> h_tpl_start;
     std::vector<int32_t> h_tpl_index;
-    std::vector<int64_t> h_scr_off;
-    int scr_arrays = 0;               // plain path: per-pixel arrays the scratch offsets are laid out for
+    std::vector<int64_t> h_scr_off;   // plain path: Σ n_px of the subsets before each (pixels; a
+                                      // subset's arrays start at that times the arrays per pixel)
+    bool arrays_valid = false;        // d_scratch holds every subset's f and grad f of the stored results
     // list solves (dic_batch_solve_list): pinned, device-mapped staging of
     // the list, its dispatch order, the initial warps and the results,
     // grown on demand to list_cap subsets
@@ -138,9 +140,13 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
 // null); p0_list / res_list (with a list): the listed subsets' initial warps
 // (12 each) and result records in list order, device-visible (the batch's
 // own p0 and results are then neither read nor written for them)
+// ext_scratch: a caller-provided per-pixel scratch of ext_doubles doubles
+// instead of the batch's own (grown on demand)
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
                       const int32_t* list = nullptr, int64_t count = 0, const double* p0_list = nullptr,
-                      dic_subset_result* res_list = nullptr);
+                      dic_subset_result* res_list = nullptr, double* ext_scratch = nullptr, size_t ext_doubles = 0);
+// per-pixel doubles of the plain kernel's scratch: f, grad f, g (+ grad g for FAGN)
+inline int plain_arrays(int method) { return method == DIC_METHOD_FAGN ? 6 : 4; }
 // icgn_batch.hip
 struct SizeClass {
     int64_t start, count;  // run of the dispatch order

@@ -45,7 +45,7 @@ EXPORTED = [
     "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
     "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
     "dic_frame_graph_destroy", "dic_batch_solve_list", "dic_mesh_solve", "dic_mesh_element_strains",
-    "dic_image_grey",
+    "dic_image_grey", "dic_batch_get_schedule",
 ]
 
 
@@ -149,6 +149,7 @@ def load():
         L.dic_batch_subset_arrays.argtypes = [_vp, ctypes.c_int64, _vp, _vp]
         L.dic_batch_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.dic_batch_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
+        L.dic_batch_get_schedule.argtypes = [_vp, _vp, _vp]
         L.dic_image_stage.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
         L.dic_batch_reset_warps.argtypes = [_vp, _vp, ctypes.c_int]
         L.dic_frame_graph_create.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(SolveParams),
@@ -375,6 +376,14 @@ class Batch:
         keep the fastest (results are unspecified until the next solve)."""
         prm = SolveParams(METHOD["icgn"], PREC_COMP_F32, int(max_iterations), 0, float(tolerance), float(max_norm))
         check(load().dic_batch_autotune(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm)))
+
+    def schedule(self):
+        """{fewest wavefronts of a size class: (wavefronts per subset, window
+        slack)} the batched engine runs each class with."""
+        w = np.zeros(4, np.int32)
+        s = np.zeros(4, np.int32)
+        check(load().dic_batch_get_schedule(self.handle, ptr(w), ptr(s)))
+        return {1 << k: (int(w[k]), int(s[k])) for k in range(4)}
 
     def solve(self, f_img, g_img, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50, tolerance=0.75,
               max_norm=1e-5, history=False):

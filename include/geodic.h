@@ -167,7 +167,9 @@ int dic_batch_results(dic_batch* b, dic_subset_result* out);
  * (iteration, norm, c_zncc, c_znssd); rows past `iterations` are zero. */
 int dic_batch_history(dic_batch* b, double* out);
 /* Reference intensities f (n_px) and gradients grad_f (n_px x 2) of one
- * subset (subset.rs:877-884); available after a DIC_PREC_PLAIN_F64 solve. */
+ * subset (subset.rs:877-884); available after a whole-batch
+ * DIC_PREC_PLAIN_F64 dic_batch_solve, until the batch's next solve of any
+ * other kind (then DIC_ERR_NOT_SOLVED). */
 int dic_batch_subset_arrays(dic_batch* b, int64_t subset, double* f, double* grad_f);
 /* Schedule of the batched engine (DIC_PREC_COMP_F32): LDS window slack in
  * pixels per side beyond the template box, and wavefronts per subset for the
@@ -183,6 +185,14 @@ int dic_batch_set_schedule(dic_batch* b, int win_slack, int waves_per_subset);
  * batch afterwards are unspecified until the next solve.  No reference
  * counterpart (the reference has no device schedule). */
 int dic_batch_autotune(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
+/* The schedule the batched engine runs each size class of the batch with:
+ * wps[k] wavefronts per subset and slack[k] pixels of window slack for the
+ * class of subsets needing 1, 2, 4 and 8 wavefronts (k = 0..3): the
+ * autotuned choice if any, else the fewest wavefronts and the batch's
+ * slack; with wavefronts set by dic_batch_set_schedule, that count for
+ * every class.  No reference counterpart (the reference has no device
+ * schedule). */
+int dic_batch_get_schedule(dic_batch* b, int32_t* wps, int32_t* slack);
 /* Solves only the listed subsets of a batch (each index at most once), each
  * from its row of p0 (count x 12, host; missing entries zero), and writes
  * their results to out (count records, host) in list order.  The batch's

@@ -85,13 +85,20 @@ def test_prefilter_constant_image():
     assert np.max(np.abs(I.coef - 77.0)) < 1e-8 * 77.0
 
 
-def test_prefilter_config_sizes():
-    # every padded length of the workloads: 2088 = 2^3·3^2·29, 1064 = 2^3·7·19
+# every padded line length of the workloads, through both evaluations:
+# 1064 = 2^3·7·19 (1024^2), 2088 = 2^3·3^2·29 (2048^2), 4136 = 2^3·11·47 and
+# 3040 = 2^5·5·19 (4096 x 3000); the thin images keep the oracle fast while
+# their long lines carry the large radix stages
+@pytest.mark.parametrize("shape", [(1024, 1024), (2048, 64), (64, 4096), (3000, 48)])
+def test_prefilter_config_sizes(shape, prefilter):
     rng = np.random.default_rng(11)
-    for shape in [(1024, 1024), (2048, 64)]:
-        img = rng.uniform(0, 255, shape)
-        C = gp.Image(image_gs=img, border=20).coef
-        np.testing.assert_allclose(C, O.get_c(img, 20), rtol=1e-8, atol=1e-9)
+    img = rng.uniform(0, 255, shape)
+    C = gp.Image(image_gs=img, border=20).coef
+    Co = O.get_c(img, 20)
+    err = np.max(np.abs(C - Co))
+    print(f"prefilter {shape} ({'fir' if prefilter == _native.PREFILTER_FIR else 'fft'}): max |dC| {err:.2e}")
+    np.testing.assert_allclose(C, Co, rtol=1e-8, atol=1e-9)
+    assert err < 1e-10
 
 
 @pytest.mark.parametrize("order", [0, 1, 2])
@@ -266,3 +273,36 @@ def test_uneven_size_classes_alone_and_in_mix(band_small):
                                  subset_order=2, precision="f32")
         assert np.array_equal(alone["p"][0], full["p"][k])
         assert alone["iterations"][0] == full["iterations"][k]
+
+
+def test_autotuned_schedule_solves_within_the_bar(band_small):
+    # autotune, then solve: every class runs one of its candidate schedules,
+    # the solve under the chosen schedule is repeatable bit for bit and
+    # agrees with the oracle to the FP32 engine's bar; a second autotune
+    # either keeps the choice (then the same results) or picks another
+    # candidate whose results meet the same bar
+    b, orc = band_small, band_small["orc"]
+    f = gp.Image(image_gs=b["ref"], border=20)
+    g = gp.Image(image_gs=b["tar"], border=20)
+    index, start, xy = gp.templates_from_masks([m.coords for m in b["local"]])
+    bt = _native.Batch(gp._engine(), b["coords"], 2, index, start, xy)
+    bt.set_p0(b["p0"])
+    assert bt.schedule() == {1: (1, 3), 2: (2, 3), 4: (4, 3), 8: (8, 3)}
+    runs = []
+    for attempt in range(2):
+        bt.autotune(f._dev, g._dev, max_norm=1e-5)
+        sched = bt.schedule()
+        for base, (w, sl) in sched.items():
+            assert w in (base, min(8, 2 * base)) and sl in (1, 3), (base, w, sl)
+        bt.solve(f._dev, g._dev, precision=_native.PREC_COMP_F32, max_norm=1e-5)
+        r = bt.results()
+        bt.solve(f._dev, g._dev, precision=_native.PREC_COMP_F32, max_norm=1e-5)
+        assert r.tobytes() == bt.results().tobytes()
+        both = (r["converged"] == 1) & (orc["converged"] == 1)
+        du = np.abs(r["p"][both, :2] - orc["p"][both, :2]).max()
+        agree = np.mean(r["converged"] == orc["converged"])
+        print(f"autotune {attempt}: schedule {sched}, max |du| {du:.2e} px, convergence agreement {agree:.4f}")
+        assert du < 1e-4 and agree >= 0.98
+        runs.append((sched, r))
+    if runs[0][0] == runs[1][0]:
+        assert runs[0][1].tobytes() == runs[1][1].tobytes()

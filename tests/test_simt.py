@@ -22,11 +22,13 @@ def driver():
     return os.path.join(HERE, "simt_driver")
 
 
-# the last two run the list kernel with four pixels per thread at eight
-# wavefronts per subset
+# the last three run the list kernel with four pixels per thread at eight
+# wavefronts per subset, the last with a window row wider than 72 floats
+# (slack 40: the instantiation with the window's own row stride)
 @pytest.mark.parametrize("order,radius,nside,slack,wps,list_mode",
                          [(0, 10, 2, 3, 1, 0), (1, 10, 2, 3, 1, 0), (2, 10, 2, 3, 1, 0), (1, 12, 1, 0, 2, 0),
-                          (1, 7, 2, 40, 1, 0), (2, 12, 1, 3, 4, 0), (1, 12, 2, 3, 8, 1), (2, 10, 1, 3, 8, 1)])
+                          (1, 7, 2, 40, 1, 0), (2, 12, 1, 3, 4, 0), (1, 12, 2, 3, 8, 1), (2, 10, 1, 3, 8, 1),
+                          (1, 12, 2, 40, 8, 1)])
 def test_batch_kernel_emulated_matches_oracle(driver, order, radius, nside, slack, wps, list_mode):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
     r = subprocess.run([driver, str(order), str(radius), str(nside), "50", str(slack), str(wps), str(list_mode)],
