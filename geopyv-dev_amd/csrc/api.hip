@@ -102,6 +102,13 @@ int dic_engine_create(int device, dic_engine** out) {
         dic_engine_destroy(d);
         return DIC_ERR_DEVICE;
     }
+    // device address of the fixed scratch of dic_batch_solve_noalloc on
+    // this device (once per device, with the device current)
+    const int sst = noalloc_scratch_init(device);
+    if (sst) {
+        dic_engine_destroy(d);
+        return sst;
+    }
     *out = d;
     return DIC_OK;
 }
@@ -625,6 +632,43 @@ int dic_batch_solve(dic_batch* db, const dic_image* f, const dic_image* g, const
     b->last_max_it = prm->max_iterations;
     b->last_history = prm->want_history != 0;
     b->arrays_valid = prm->precision == DIC_PREC_PLAIN_F64;
+    return DIC_OK;
+}
+
+int dic_batch_solve_noalloc(dic_batch* db, const dic_image* f, const dic_image* g, const dic_solve_params* prm) {
+    CHECK_ARG(db && f && g && prm, "dic_batch_solve_noalloc: NULL argument");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    const int vst = validate_solve(b, &f->im, &g->im, prm);
+    if (vst) return vst;
+    CHECK_ARG(prm->precision == DIC_PREC_PLAIN_F64, "invalid input: dic_batch_solve_noalloc runs DIC_PREC_PLAIN_F64");
+    CHECK_ARG(!prm->want_history, "invalid input: dic_batch_solve_noalloc records no history");
+    const size_t need = (size_t)b->total_px * plain_arrays(prm->method) * sizeof(double);
+    if (need > kNoallocScratchBytes) {
+        set_error("invalid input: dic_batch_solve_noalloc: the batch's per-pixel arrays need " + std::to_string(need) +
+                  " bytes, beyond the fixed scratch of " + std::to_string(kNoallocScratchBytes) + " bytes");
+        return DIC_ERR_INVALID_INPUT;
+    }
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    double* ptr = noalloc_scratch(e->device);
+    if (!ptr) {
+        set_error("dic_batch_solve_noalloc: the fixed scratch was not set up on this device");
+        return DIC_ERR_DEVICE;
+    }
+    if (!e->capturing) DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
+    int st = launch_icgn_plain(b, &f->im, &g->im, prm, nullptr, 0, nullptr, nullptr, ptr,
+                               kNoallocScratchBytes / sizeof(double));
+    if (st) return st;
+    if (!e->capturing) {
+        DIC_HIP_TRY(hipEventRecord(e->ev_sol1, e->stream));
+        e->sol_recorded = true;
+    }
+    b->last_precision = prm->precision;
+    b->last_method = prm->method;
+    b->last_max_it = prm->max_iterations;
+    b->last_history = false;
+    b->arrays_valid = false;  // the per-pixel arrays stay in the shared scratch
     return DIC_OK;
 }
 

@@ -612,4 +612,42 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     return DIC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of dic_batch_solve_noalloc: a file-scope device array that
+// every GPU the library runs on holds for the life of the process.  In host
+// code `scratch` names a host-side stand-in, not a device address, so host
+// code never passes it to a kernel or a copy: the device address of each
+// GPU's own copy is looked up once per device (noalloc_scratch_init, called
+// by dic_engine_create with the device current) and used from then on.
+// ---------------------------------------------------------------------------
+namespace {
+__device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device index beyond the library's device table");
+        return DIC_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return DIC_OK;
+    void* ptr = nullptr;
+    DIC_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+        set_error("hipGetSymbolAddress(scratch) returned a null address");
+        return DIC_ERR_DEVICE;
+    }
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return DIC_OK;
+}
+
+double* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 }  // namespace geodic

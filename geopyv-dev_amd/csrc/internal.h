@@ -147,6 +147,12 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
                       dic_subset_result* res_list = nullptr, double* ext_scratch = nullptr, size_t ext_doubles = 0);
 // per-pixel doubles of the plain kernel's scratch: f, grad f, g (+ grad g for FAGN)
 inline int plain_arrays(int method) { return method == DIC_METHOD_FAGN ? 6 : 4; }
+// The library's fixed device scratch for dic_batch_solve_noalloc: its
+// device address on the current device, looked up once per device (at
+// engine creation); kNoallocScratchBytes is its size.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+int noalloc_scratch_init(int device);
+double* noalloc_scratch(int device);
 // icgn_batch.hip
 struct SizeClass {
     int64_t start, count;  // run of the dispatch order

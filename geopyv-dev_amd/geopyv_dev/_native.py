@@ -45,7 +45,7 @@ EXPORTED = [
     "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
     "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
     "dic_frame_graph_destroy", "dic_batch_solve_list", "dic_mesh_solve", "dic_mesh_element_strains",
-    "dic_image_grey", "dic_batch_get_schedule",
+    "dic_image_grey", "dic_batch_solve_noalloc", "dic_batch_get_schedule",
 ]
 
 
@@ -144,6 +144,7 @@ def load():
                                        ctypes.POINTER(_vp)]
         L.dic_batch_set_p0.argtypes = [_vp, _vp, ctypes.c_int]
         L.dic_batch_solve.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
+        L.dic_batch_solve_noalloc.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
         L.dic_batch_results.argtypes = [_vp, _vp]
         L.dic_batch_history.argtypes = [_vp, _vp]
         L.dic_batch_subset_arrays.argtypes = [_vp, ctypes.c_int64, _vp, _vp]
@@ -391,6 +392,14 @@ class Batch:
                           float(tolerance), float(max_norm))
         self.max_iterations = int(max_iterations)
         check(load().dic_batch_solve(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm)))
+
+    def solve_noalloc(self, f_img, g_img, method="icgn", max_iterations=50, tolerance=0.75, max_norm=1e-5):
+        """The plain FP64 solve through dic_batch_solve_noalloc: its per-pixel
+        scratch is the library's fixed device array, so it allocates nothing
+        (batches beyond that array's size raise GeodicError)."""
+        prm = SolveParams(METHOD[method], PREC_PLAIN_F64, int(max_iterations), 0, float(tolerance), float(max_norm))
+        self.max_iterations = int(max_iterations)
+        check(load().dic_batch_solve_noalloc(self.handle, f_img.handle, g_img.handle, ctypes.byref(prm)))
 
     def solve_list(self, f_img, g_img, idx, p0, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50,
                    tolerance=0.75, max_norm=1e-5):

@@ -161,6 +161,19 @@ int dic_batch_set_p0(dic_batch* b, const double* p0, int mem);
  * engine has been synchronised since it last wrote them (a shared cache of
  * reference images serving several engines). */
 int dic_batch_solve(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): dic_batch_solve for DIC_PREC_PLAIN_F64,
+ * want_history 0, whose per-pixel scratch (f, grad f and g per pixel, plus
+ * grad g for FAGN: 32 or 48 bytes per template pixel summed over the batch)
+ * lives in a fixed 64 MiB device array of the library instead of memory the
+ * batch allocates; a larger batch is rejected with DIC_ERR_INVALID_INPUT.
+ * It allocates nothing (the batch and images are allocated when created).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several engines or threads) race; each call writes every scratch
+ * element before reading it.  Results as dic_batch_solve; per-pixel arrays
+ * (dic_batch_subset_arrays) are not kept. */
+int dic_batch_solve_noalloc(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const dic_solve_params* prm);
 /* Waits, then copies n_subsets results to the host. */
 int dic_batch_results(dic_batch* b, dic_subset_result* out);
 /* Waits, then copies n_subsets*max_iterations*4 history doubles
