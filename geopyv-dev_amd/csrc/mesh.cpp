@@ -443,6 +443,19 @@ struct Run {
     std::vector<int64_t> retry;
     std::vector<int32_t> cand_pos;  // node -> its index among the current candidates, else -1
     Workers workers;
+    const dic_progress* pg = nullptr;  // observer (may be null)
+
+    bool cancelled() const { return pg && pg->cancelled && pg->cancelled(pg->user) != 0; }
+    // progress::node_done (progress.rs:34-37): report a stored node, then
+    // honour a cancellation request
+    int node_done() const {
+        if (pg && pg->node_done) pg->node_done(pg->user);
+        if (cancelled()) {
+            set_error("solve cancelled");
+            return DIC_ERR_CANCELLED;
+        }
+        return DIC_OK;
+    }
 
     // the node's trusted neighbours, in a buffer reused per host thread
     const std::vector<int64_t>& trusted(int64_t i, const uint8_t* skip) const {
@@ -518,12 +531,17 @@ struct Run {
         norms[i] = r.iterations > 0 ? r.norm : 0.0;
     }
 
-    void fold(const std::vector<int64_t>& nodes_in, const std::vector<dic_subset_result>& kept) {
+    // stores the nodes in the given (ascending) order, each reported to the
+    // observer, which may stop the solve after any of them
+    int fold(const std::vector<int64_t>& nodes_in, const std::vector<dic_subset_result>& kept) {
         for (size_t k = 0; k < nodes_in.size(); ++k) {
             store(nodes_in[k], kept[k]);
+            const int st = node_done();
+            if (st) return st;
             stored[nodes_in[k]] = 1;
             heap.push({heap_key(czn[nodes_in[k]]), nodes_in[k]});
         }
+        return DIC_OK;
     }
 
     // one node at a time (serial RG and the corrections)
@@ -542,9 +560,9 @@ struct Run {
             const int64_t nb = *q;
             if (stored[nb]) continue;
             dic_subset_result r;
-            const int st = solve_one(nb, trusted(nb, nullptr), sp.tolerance, r);
+            int st = solve_one(nb, trusted(nb, nullptr), sp.tolerance, r);
+            if (!st) st = fold({nb}, {r});
             if (st) return st;
-            fold({nb}, {r});
         }
         return DIC_OK;
     }
@@ -594,9 +612,9 @@ struct Run {
                 initial_warp(nodes, &nodes[2 * members[k]], trusted(members[k], nullptr), czn.data(), P.data(), plen,
                              &pa[k * plen]);
             });
-            const int st = solve_nodes(members, pa, sp.tolerance, kept);
+            int st = solve_nodes(members, pa, sp.tolerance, kept);
+            if (!st) st = fold(members, kept);
             if (st) return st;
-            fold(members, kept);
         }
         return DIC_OK;
     }
@@ -650,6 +668,12 @@ struct Run {
         std::vector<double> pa;
         std::vector<dic_subset_result> kept;
         for (int32_t w = 0; w < n_waves; ++w) {
+            // the corrections report no nodes (mesh.rs:1265-1414), but a
+            // cancellation is honoured at every list boundary
+            if (cancelled()) {
+                set_error("solve cancelled");
+                return DIC_ERR_CANCELLED;
+            }
             members.clear();
             for (size_t i = 0; i < seq.size(); ++i)
                 if (wave[i] == w) members.push_back(seq[i]);
@@ -819,7 +843,7 @@ int dic_mesh_element_strains(int64_t n_nodes, const double* nodes, int64_t n_ele
 int dic_mesh_solve(dic_engine* d, const dic_image* f, const dic_image* g, int64_t n_nodes, const double* nodes,
                    int64_t n_elements, int32_t mesh_order, const int64_t* elements, const int32_t* tpl_index,
                    int32_t n_templates, const int64_t* tpl_start, const int16_t* tpl_xy, const dic_mesh_params* prm,
-                   dic_mesh_output* out) {
+                   const dic_progress* progress, dic_mesh_output* out) {
     if (!d || !f || !g || !prm || !out || !out->p || !out->c_zncc || !out->iterations || !out->norms ||
         !out->quality_ok || !out->warps) {
         set_error("dic_mesh_solve: NULL argument");
@@ -872,6 +896,8 @@ int dic_mesh_solve(dic_engine* d, const dic_image* f, const dic_image* g, int64_
     R.disp.assign(2 * n_nodes, 0.0);
     R.norms.assign(n_nodes, 0.0);
     R.iters.assign(n_nodes, 0);
+    R.pg = progress;
+    if (progress && progress->mesh_start) progress->mesh_start(progress->user, n_nodes);
 
     // seed node (mesh.rs:659-672): the node nearest the seed coordinate,
     // the user's warp, the seed tolerance
@@ -896,8 +922,8 @@ int dic_mesh_solve(dic_engine* d, const dic_image* f, const dic_image* g, int64_
             std::vector<double> pa(prm->seed_warp, prm->seed_warp + plen);
             st = R.solve_nodes({seed}, pa, prm->seed_tolerance, kept);
         }
+        if (!st) st = R.fold({seed}, kept);
         if (!st) {
-            R.fold({seed}, kept);
             R.propagated[seed] = 1;
             if (prm->preconditioning == DIC_PRECOND_RG) {
                 // serial cascade, highest C_ZNCC first (mesh.rs:678-700)

@@ -85,6 +85,40 @@ class MeshOutput(ctypes.Structure):
     ]
 
 
+_MESH_START = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int64)
+_NODE_DONE = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+_CANCELLED = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p)
+
+
+class Progress(ctypes.Structure):
+    """dic_progress: the SolveProgress observer (src/progress.rs:16-38)."""
+    _fields_ = [("user", ctypes.c_void_p), ("mesh_start", _MESH_START), ("node_done", _NODE_DONE),
+                ("cancelled", _CANCELLED)]
+
+
+def progress_struct(observer):
+    """A dic_progress calling observer.mesh_start(n), observer.node_done()
+    and observer.cancelled() (each optional, as the trait's no-op
+    defaults).  Returns (struct, keepalive): the callbacks must outlive the
+    call."""
+    if observer is None:
+        return None, None
+    cbs = []
+
+    def wrap(name, ctype, fn):
+        if not callable(getattr(observer, name, None)):
+            return ctype()
+        c = ctype(fn)
+        cbs.append(c)
+        return c
+
+    pr = Progress(None,
+                  wrap("mesh_start", _MESH_START, lambda _u, n: observer.mesh_start(int(n))),
+                  wrap("node_done", _NODE_DONE, lambda _u: observer.node_done()),
+                  wrap("cancelled", _CANCELLED, lambda _u: 1 if observer.cancelled() else 0))
+    return pr, cbs
+
+
 RESULT_DTYPE = np.dtype([
     ("p", np.float64, (12,)),
     ("c_zncc", np.float64), ("c_znssd", np.float64), ("norm", np.float64),
@@ -108,6 +142,15 @@ class GeodicError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(msg)
         self.status = status
+
+
+class SolveCancelled(RuntimeError):
+    """A solve stopped by its progress observer (Error::Cancelled,
+    error.rs:38-40; raised as RuntimeError("solve cancelled") like every
+    Error the PyO3 layer surfaces)."""
+
+    def __init__(self, msg="solve cancelled"):
+        super().__init__(msg)
 
 
 def load():
@@ -165,7 +208,7 @@ def load():
                                            _vp]
         L.dic_mesh_solve.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
                                      ctypes.c_int32, _vp, _vp, ctypes.POINTER(MeshParams),
-                                     ctypes.POINTER(MeshOutput)]
+                                     ctypes.POINTER(Progress), ctypes.POINTER(MeshOutput)]
         L.dic_mesh_element_strains.argtypes = [ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp]
         _lib = L
         return L
@@ -489,10 +532,12 @@ def mesh_element_strains(nodes, elements, displacements, mesh_order):
 def mesh_solve(engine, f_img, g_img, nodes, elements, mesh_order, tpl_index, tpl_start, tpl_xy, seed_coord,
                seed_warp=None, subset_order=1, method="icgn", precision=PREC_PLAIN_F64, max_iterations=50,
                tolerance=0.75, max_norm=1e-5, seed_tolerance=0.9, preconditioning="RG", override=False,
-               batch_factor=4, root_rel_eps=0.02, workers=0):
+               batch_factor=4, root_rel_eps=0.02, workers=0, progress=None, out=None):
     """dic_mesh_solve.  Returns (status, message, outputs dict); outputs
     are filled when the propagation completed (status 0, or the quality
-    gate, a degenerate or a folded element)."""
+    gate, a degenerate or a folded element).  progress: an observer with
+    any of mesh_start(n), node_done(), cancelled() (SolveProgress); out: a
+    dict of caller-owned output arrays to fill instead of new ones."""
     L = load()
     nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 2)
     el = np.ascontiguousarray(elements, dtype=np.int64)
@@ -517,14 +562,28 @@ def mesh_solve(engine, f_img, g_img, nodes, elements, mesh_order, tpl_index, tpl
     for k in range(12):
         prm.seed_warp[k] = sw[k]
     prm.seed_tolerance = float(seed_tolerance)
-    res = dict(p=np.zeros((n, 12)), c_zncc=np.zeros(n), iterations=np.zeros(n, dtype=np.int32), norms=np.zeros(n),
-               quality_ok=np.zeros(n, dtype=np.int32), warps=np.zeros((ne, 12)))
-    out = MeshOutput(res["p"].ctypes.data, res["c_zncc"].ctypes.data, res["iterations"].ctypes.data,
-                     res["norms"].ctypes.data, res["quality_ok"].ctypes.data, res["warps"].ctypes.data, -1, 0, 0, 0, 0.0)
+    if out is None:
+        res = dict(p=np.zeros((n, 12)), c_zncc=np.zeros(n), iterations=np.zeros(n, dtype=np.int32),
+                   norms=np.zeros(n), quality_ok=np.zeros(n, dtype=np.int32), warps=np.zeros((ne, 12)))
+    else:
+        res = out
+        for k, shape, dt in (("p", (n, 12), np.float64), ("c_zncc", (n,), np.float64),
+                             ("iterations", (n,), np.int32), ("norms", (n,), np.float64),
+                             ("quality_ok", (n,), np.int32), ("warps", (ne, 12), np.float64)):
+            a = res[k]
+            if a.shape != shape or a.dtype != dt or not a.flags.c_contiguous:
+                raise ValueError(f"out[{k!r}] must be a C-contiguous {dt.__name__} array of shape {shape}")
+    mo = MeshOutput(res["p"].ctypes.data, res["c_zncc"].ctypes.data, res["iterations"].ctypes.data,
+                    res["norms"].ctypes.data, res["quality_ok"].ctypes.data, res["warps"].ctypes.data, -1, 0, 0, 0,
+                    0.0)
+    pr, keep = progress_struct(progress)
     st = L.dic_mesh_solve(engine.handle, f_img.handle, g_img.handle, n, ptr(nodes), ne, int(mesh_order), ptr(el),
                           ptr(tpl_index), len(tpl_start) - 1, ptr(tpl_start), ptr(tpl_xy), ctypes.byref(prm),
-                          ctypes.byref(out))
+                          ctypes.byref(pr) if pr is not None else None, ctypes.byref(mo))
+    del keep
+    out = mo
     msg = L.dic_last_error().decode(errors="replace") if st else ""
+    res = dict(res)
     res["p"] = res["p"][:, :6 * int(subset_order)]
     res["quality_ok"] = res["quality_ok"].astype(bool)
     res.update(seed_node=int(out.seed_node), n_node_solves=int(out.n_node_solves), n_lists=int(out.n_lists),

@@ -215,11 +215,15 @@ class Mesh:
 
     def solve(self, local_mask, seed_coord, seed_warp=None, solver=None, solver_options=None, max_norm=1e-5,
               max_iterations=50, subset_order=1, tolerance=0.75, seed_tolerance=0.9, override_active=False,
-              precision="f64", **kwargs):
+              precision="f64", progress=None, **kwargs):
         """Seed, RG or layer-RG propagation, corrections, the quality gate,
         element strains and the compatibility check (mesh.rs:628-857).
         ``precision`` "f64" runs the FP64 drop-in solver at every node, "f32"
-        the batched engine (FP32 interpolation, FP64 accumulation)."""
+        the batched engine (FP32 interpolation, FP64 accumulation).
+        ``progress`` is Mesh::solve's observer (mesh.rs:610-616): an object
+        with any of ``mesh_start(n)``, ``node_done()`` and ``cancelled()``;
+        a cancelled solve raises RuntimeError("cancelled") and stores
+        nothing."""
         from . import Mask, _checked_solver, _resolve_solver
         if not isinstance(local_mask, Mask) or local_mask.mask_type != "local":
             raise TypeError("local_mask must be a Mask")
@@ -258,7 +262,10 @@ class Mesh:
             precision=_native.PREC_PLAIN_F64 if precision == "f64" else _native.PREC_COMP_F32,
             max_iterations=max_iterations, tolerance=tolerance, max_norm=max_norm, seed_tolerance=seed_tolerance,
             preconditioning=pre, override=override_active, batch_factor=batch_factor,
-            root_rel_eps=float(lrg.get("root_rel_eps", 0.02)), workers=int(lrg.get("max_workers") or 0))
+            root_rel_eps=float(lrg.get("root_rel_eps", 0.02)), workers=int(lrg.get("max_workers") or 0),
+            progress=progress)
+        if st == _native.ERR_CANCELLED:
+            raise _native.SolveCancelled(msg or "solve cancelled")
         if st:
             raise RuntimeError(msg or f"mesh solve failed with status {st}")
         p = r["p"]

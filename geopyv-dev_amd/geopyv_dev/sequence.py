@@ -17,6 +17,7 @@ import re
 
 import numpy as np
 
+from . import _native
 from .mesh import element_centroids
 
 _IMAGE_EXTS = (".jpg", ".jpeg", ".png")
@@ -161,12 +162,24 @@ class Sequence:
             return Image(filepath=fr, border=border)
         return Image(image_gs=np.asarray(fr, dtype=np.float64), border=border)
 
+    def _label(self, k):
+        """A frame's name in pair labels: its file stem, else its index."""
+        fr = self._frames[k]
+        if isinstance(fr, (str, os.PathLike)):
+            return os.path.splitext(os.path.basename(os.fspath(fr)))[0]
+        return str(k)
+
     def solve(self, local_mask, seed_coord, seed_warp=None, solver=None, solver_options=None, max_norm=1e-5,
               max_iterations=50, subset_order=1, tolerance=0.75, seed_tolerance=0.9, guide=True, sequential=False,
-              sync=True, override=False, border=20, precision="f64", **kwargs):
+              sync=True, override=False, border=20, precision="f64", progress=None, **kwargs):
         """The frame loop of sequence.rs:624-968 for uniform masking,
         deformable texture and uniform topology, on this sequence's mesh
-        (``sync`` is implied: the geometry is reused for every pair)."""
+        (``sync`` is implied: the geometry is reused for every pair).
+        ``progress`` is Sequence::solve's observer: ``pair_start(pair,
+        n_pairs, label)`` before every pair (sequence.rs:788-791), then the
+        mesh solve's ``mesh_start``/``node_done``/``cancelled``; a
+        cancellation raises RuntimeError("solve cancelled") and stores
+        nothing (it is not a failed pair to fall back from)."""
         from . import Mask, _checked_solver, _resolve_solver
         from .mesh import Mesh
         if not isinstance(local_mask, Mask) or local_mask.mask_type != "local":
@@ -193,13 +206,20 @@ class Sequence:
                     exclusions=self._exclusions, exclusions_hard=self._exclusions_hard)
         while True:
             mesh._use_images(f_img, g_img)
+            if progress is not None:
+                if callable(getattr(progress, "pair_start", None)):
+                    progress.pair_start(len(solutions), n_pairs, f"{self._label(f_index)}→{self._label(g_index)}")
+                if callable(getattr(progress, "cancelled", None)) and progress.cancelled():
+                    raise _native.SolveCancelled()
             # override: tolerance 0 and the quality gate bypassed (sequence.rs:751-766)
             tol = 0.0 if mesh_override else tolerance
             try:
                 mesh.solve(local_mask, seed_coord=coord, seed_warp=warp, solver=method,
                            solver_options=solver_options, max_norm=max_norm, max_iterations=max_iterations,
                            subset_order=subset_order, tolerance=tol, seed_tolerance=seed_tolerance,
-                           override_active=mesh_override, precision=precision)
+                           override_active=mesh_override, precision=precision, progress=progress)
+            except _native.SolveCancelled:
+                raise
             except RuntimeError:
                 if f_index + 1 < g_index:
                     # a non-consecutive pair failed: the previous frame becomes

@@ -280,16 +280,38 @@ typedef struct {
     int64_t n_unsolvable;    /* nodes without quality_ok after corrections */
     double list_seconds;     /* wall time spent in the list solves (the rest is host traversal) */
 } dic_mesh_output;
+/* Progress and cancellation observer of a mesh solve: the SolveProgress
+ * trait (src/progress.rs:16-38) that Mesh::solve takes (mesh.rs:610-616).
+ * Every callback is made from the thread that called dic_mesh_solve, never
+ * from the device or a worker; any pointer may be NULL.
+ *   mesh_start(user, n)  a mesh solve over n nodes is starting;
+ *   node_done(user)      one more node has been solved and stored (the
+ *                        seed, then every node of the propagation, in the
+ *                        order they are stored: mesh.rs:674,1143,1246);
+ *   cancelled(user)      polled after every node_done and before every list
+ *                        solve of the corrections; nonzero stops the solve
+ *                        with DIC_ERR_CANCELLED and nothing written to the
+ *                        outputs (mesh.rs:608-609).
+ * The sequence loop's pair_start (sequence.rs:788) belongs to the caller's
+ * frame loop above this ABI. */
+typedef struct {
+    void* user;
+    void (*mesh_start)(void* user, int64_t n_nodes);
+    void (*node_done)(void* user);
+    int32_t (*cancelled)(void* user);
+} dic_progress;
 /* Seed, RG or layer-RG propagation, corrections, the quality gate, element
  * strains and the compatibility check.  Returns DIC_ERR_INVALID_INPUT for
  * an unsolvable mesh (unless override_active) or a folded element
  * (det F <= 0), DIC_ERR_MESH_GENERATION for a degenerate element, a subset
- * status for a failed node solve (as the reference's `?` does); the
- * outputs are filled whenever the propagation itself completed. */
+ * status for a failed node solve (as the reference's `?` does), and
+ * DIC_ERR_CANCELLED when `progress` asks for it; the outputs are filled
+ * whenever the propagation itself completed.  progress may be NULL (no
+ * observer). */
 int dic_mesh_solve(dic_engine* e, const dic_image* f_img, const dic_image* g_img, int64_t n_nodes,
                    const double* nodes, int64_t n_elements, int32_t mesh_order, const int64_t* elements,
                    const int32_t* tpl_index, int32_t n_templates, const int64_t* tpl_start, const int16_t* tpl_xy,
-                   const dic_mesh_params* prm, dic_mesh_output* out);
+                   const dic_mesh_params* prm, const dic_progress* progress, dic_mesh_output* out);
 /* element_strains (mesh.rs:1552-1665) alone, host-side: warps
  * (n_elements x 12) from node displacements (n_nodes x 2). */
 int dic_mesh_element_strains(int64_t n_nodes, const double* nodes, int64_t n_elements, int32_t mesh_order,

@@ -245,3 +245,72 @@ def test_drop_in_mesh_class(band):
     with pytest.raises(RuntimeError, match="mesh unsolvable"):
         m.solve(gp.Mask("local", shape="circle", size=6), seed_coord=[50.0, 30.0], tolerance=0.99995,
                 seed_tolerance=0.5)
+
+
+class _Counting:
+    """SolveProgress observer counting node_done calls and cancelling once
+    stop_after have been reported (mesh.rs:2995-3012)."""
+
+    def __init__(self, stop_after=None):
+        self.stop_after = stop_after
+        self.started = []
+        self.done = 0
+        self.polls = 0
+
+    def mesh_start(self, n):
+        self.started.append(n)
+
+    def node_done(self):
+        self.done += 1
+
+    def cancelled(self):
+        self.polls += 1
+        return self.stop_after is not None and self.done >= self.stop_after
+
+
+@pytest.mark.parametrize("pre", ["RG", "layer-RG"])
+def test_progress_reports_every_node_and_cancels(band, pre):
+    # mesh.rs:3013-3036 through the C ABI: uncancelled, one mesh_start(n)
+    # and at least n node_done calls with results equal to a solve without
+    # an observer; cancelled after 5 (and, for layer-RG, after 37: inside
+    # a colour's list) nodes, status DIC_ERR_CANCELLED, exactly that many
+    # node_done calls and the outputs untouched
+    c = band
+    xy, start = circle_template(c["radius"])
+    tpl = np.zeros(len(c["nodes"]), np.int32)
+    args = (_native.Engine.get(0), c["f"], c["g"], c["nodes"], c["el"], 1, tpl, start, np.rint(xy).astype(np.int16),
+            [50.0, 50.0], np.zeros(6))
+    kw = dict(tolerance=0.5, seed_tolerance=0.5, override=True, preconditioning=pre)
+    n = len(c["nodes"])
+    obs = _Counting()
+    st, _, r_obs = _native.mesh_solve(*args, progress=obs, **kw)
+    st0, _, r_none = _native.mesh_solve(*args, **kw)
+    assert st == 0 and st0 == 0
+    assert obs.started == [n] and obs.done >= n and obs.polls >= obs.done
+    for k in ("p", "c_zncc", "iterations", "norms", "quality_ok", "warps"):
+        np.testing.assert_array_equal(r_obs[k], r_none[k])
+    for stop in ([5, 37] if pre == "layer-RG" else [5]):
+        sentinel = dict(p=np.full((n, 12), 7.0), c_zncc=np.full(n, 7.0), iterations=np.full(n, 7, np.int32),
+                        norms=np.full(n, 7.0), quality_ok=np.full(n, 7, np.int32),
+                        warps=np.full((len(c["el"]), 12), 7.0))
+        obs = _Counting(stop_after=stop)
+        st, msg, _ = _native.mesh_solve(*args, progress=obs, out=sentinel, **kw)
+        print(f"{pre}: cancelled after {obs.done} nodes: status {st} ({msg})", flush=True)
+        assert st == _native.ERR_CANCELLED and msg == "solve cancelled"
+        assert obs.done == stop and obs.started == [n]
+        for k, a in sentinel.items():
+            assert np.all(a == 7), k
+
+
+def test_mesh_class_cancel_stores_nothing(speckle):
+    c = speckle
+    m = gp.Mesh(f_img=gp.Image(image_gs=c["ref"], border=20), g_img=gp.Image(image_gs=c["tar"], border=20),
+                nodes=c["nodes"], elements=c["el"])
+    with pytest.raises(RuntimeError, match="solve cancelled"):
+        m.solve(gp.Mask("local", shape="circle", size=20), seed_coord=[256.0, 256.0], tolerance=0.5,
+                seed_tolerance=0.5, progress=_Counting(stop_after=3))
+    assert not m.solved
+    obs = _Counting()
+    m.solve(gp.Mask("local", shape="circle", size=20), seed_coord=[256.0, 256.0], tolerance=0.5, seed_tolerance=0.5,
+            progress=obs)
+    assert m.solved and obs.started == [len(c["nodes"])] and obs.done >= len(c["nodes"])

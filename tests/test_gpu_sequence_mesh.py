@@ -139,3 +139,40 @@ def test_sequence_from_files_through_the_device_loader(tmp_path):
                              tolerance=0.75, seed_tolerance=0.75)
         assert st == 0
         assert np.abs(s.p - r["p"]).max() <= 1e-4
+
+
+class _PairCounter:
+    """sequence.rs:1695-1712: records pair_start calls and node_done
+    counts; cancels once the pair cancel_at has started."""
+
+    def __init__(self, cancel_at=None):
+        self.pairs, self.labels, self.nodes, self.cancel_at = [], [], 0, cancel_at
+
+    def node_done(self):
+        self.nodes += 1
+
+    def pair_start(self, pair, n_pairs, label):
+        self.pairs.append((pair, n_pairs))
+        self.labels.append(label)
+
+    def cancelled(self):
+        return self.cancel_at is not None and len(self.pairs) > self.cancel_at
+
+
+def test_sequence_reports_pairs_and_cancels():
+    # sequence.rs:1714-1738: pairs (0, 2), (1, 2) reported with node counts;
+    # cancelled as pair 1 starts: RuntimeError("solve cancelled"), not the
+    # failed-pair fallback, and no solution stored
+    frames = _mixtures([0.0, 0.1, 0.2])
+    nodes, el = _mesh()
+    mask = gp.Mask("local", shape="circle", size=RADIUS)
+    seq = gp.Sequence(frames=frames, nodes=nodes, elements=el)
+    p = _PairCounter()
+    seq.solve(mask, seed_coord=[128.0, 128.0], tolerance=0.75, seed_tolerance=0.75, progress=p)
+    assert p.pairs == [(0, 2), (1, 2)] and p.labels == ["0→1", "0→2"]
+    assert p.nodes >= 2 * len(nodes) and len(seq.mesh_solutions) == 2
+    seq = gp.Sequence(frames=frames, nodes=nodes, elements=el)
+    p = _PairCounter(cancel_at=1)
+    with pytest.raises(RuntimeError, match="solve cancelled"):
+        seq.solve(mask, seed_coord=[128.0, 128.0], tolerance=0.75, seed_tolerance=0.75, progress=p)
+    assert len(p.pairs) == 2 and not seq.solved
