@@ -293,6 +293,11 @@ void dic_image_destroy(dic_image* di) {
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+// Bytes of the batched engine's per-subset start record (batch::SubMeta:
+// centre x, y as doubles, template start int64, size and index int32,
+// extents 4 x int32).
+constexpr size_t kMetaBytes = 48;
+
 // Staging layout of a list solve of up to `cap` subsets: the list, its
 // dispatch order (int32 each, 8-byte aligned), the initial warps, then (at
 // list_io_bytes) the results.
@@ -309,6 +314,7 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_tpl_xy, sizeof(int32_t) * (b->h_tpl_start.empty() ? 0 : b->h_tpl_start.back()));
     dev_free(e, b->d_tpl_xy_eng, sizeof(int32_t) * (b->h_tpl_start.empty() ? 0 : b->h_tpl_start.back()));
     dev_free(e, b->d_tpl_ext, sizeof(int32_t) * 4 * b->n_tpl);
+    dev_free(e, b->d_meta, kMetaBytes * n);
     dev_free(e, b->d_scr_off, sizeof(int64_t) * n);
     dev_free(e, b->d_p0, sizeof(double) * 12 * n);
     dev_free(e, b->d_p, sizeof(double) * 12 * n);
@@ -321,13 +327,15 @@ static void batch_free(Batch* b) {
 
 namespace {
 
-// Pixel order of a template for the batched engine.  A wavefront gathers
-// 64 pixels' coefficient blocks at once from an LDS window whose rows are 72
-// floats apart, so pixel (dx, dy) starts in bank (dx + 8 dy) mod 64 (72 = 8
-// mod 64).  Whole 8x8 tiles of the template give 64 distinct banks (and
-// keep them under a near-uniform warp); the pixels of partial tiles are
-// dealt into groups of 64 taking one pixel per bank, fullest banks first.
-// Sums over the subset do not depend on the order beyond rounding.
+// Pixel order of a template for the batched engine.  The window rows in
+// LDS are 72 floats apart and LDS serves a wavefront's reads in two halves
+// of 32 lanes over 32 banks, so pixel (dx, dy) starts in bank
+// (dx + 8 dy) mod 32 and each run of 32 consecutive pixels (one half-wave)
+// should cover 32 distinct banks.  Whole 8x4 blocks of the template do
+// (and keep a compact footprint, so a near-uniform warp shifts them
+// together); the pixels of partial blocks are dealt into runs of 32 taking
+// one pixel per bank class, fullest classes first.  Sums over the subset
+// do not depend on the order beyond rounding.
 void engine_pixel_order(const int32_t* in, int64_t n, int32_t* out) {
     auto ux = [](int32_t v) { return (int)(int16_t)(v & 0xffff); };
     auto uy = [](int32_t v) { return (int)(int16_t)((uint32_t)v >> 16); };
@@ -337,28 +345,26 @@ void engine_pixel_order(const int32_t* in, int64_t n, int32_t* out) {
         x0 = std::min(x0, ux(in[i]));
         y0 = std::min(y0, uy(in[i]));
     }
-    std::map<std::pair<int, int>, std::vector<int32_t>> tiles;
-    for (int64_t i = 0; i < n; ++i) tiles[{(uy(in[i]) - y0) >> 3, (ux(in[i]) - x0) >> 3}].push_back(in[i]);
+    std::map<std::pair<int, int>, std::vector<int32_t>> blocks;
+    for (int64_t i = 0; i < n; ++i) blocks[{(uy(in[i]) - y0) >> 2, (ux(in[i]) - x0) >> 3}].push_back(in[i]);
     int64_t k = 0;
-    std::vector<std::vector<int32_t>> bank(64);
-    for (auto& kv : tiles) {
-        if (kv.second.size() == 64) {
+    std::vector<std::vector<int32_t>> bank(32);
+    for (auto& kv : blocks) {
+        if (kv.second.size() == 32) {
             for (int32_t v : kv.second) out[k++] = v;
         } else {
-            for (int32_t v : kv.second) bank[(unsigned)(ux(v) + 8 * uy(v)) & 63].push_back(v);
+            for (int32_t v : kv.second) bank[(unsigned)(ux(v) + 8 * uy(v)) & 31].push_back(v);
         }
     }
     for (auto& b : bank) std::reverse(b.begin(), b.end());
     while (k < n) {
-        std::vector<int> ord(64);
+        std::vector<int> ord(32);
         std::iota(ord.begin(), ord.end(), 0);
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return bank[a].size() > bank[c].size(); });
-        int taken = 0;
         for (int b : ord)
-            if (!bank[b].empty() && taken < 64) {
+            if (!bank[b].empty()) {
                 out[k++] = bank[b].back();
                 bank[b].pop_back();
-                ++taken;
             }
     }
 }
@@ -455,6 +461,7 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     A((void**)&b->d_p, sizeof(double) * 12 * n_subsets);
     A((void**)&b->d_res, sizeof(dic_subset_result) * n_subsets);
     A((void**)&b->d_order_perm, sizeof(int32_t) * n_subsets);
+    A(&b->d_meta, kMetaBytes * n_subsets);
     auto H2D = [&](void* dst, const void* src, size_t bytes) {
         if (!st && bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess) {
             set_error("dic_batch_create: upload failed");
@@ -473,6 +480,21 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     H2D(b->d_tpl_ext, ext.data(), sizeof(int32_t) * 4 * n_templates);
     H2D(b->d_order_perm, perm.data(), sizeof(int32_t) * n_subsets);
     H2D(b->d_scr_off, b->h_scr_off.data(), sizeof(int64_t) * n_subsets);
+    // the batched engine's start record per subset (centre, template
+    // start, size and extents)
+    std::vector<char> meta(kMetaBytes * n_subsets);
+    for (int64_t s = 0; s < n_subsets; ++s) {
+        char* m = meta.data() + kMetaBytes * s;
+        const int t = tpl_index[s];
+        const int64_t st0 = tpl_start[t];
+        const int32_t nt = (int32_t)(tpl_start[t + 1] - tpl_start[t]);
+        std::memcpy(m, &coords[2 * s], 16);
+        std::memcpy(m + 16, &st0, 8);
+        std::memcpy(m + 24, &nt, 4);
+        std::memcpy(m + 28, &t, 4);
+        std::memcpy(m + 32, &ext[4 * (size_t)t], 16);
+    }
+    H2D(b->d_meta, meta.data(), meta.size());
     if (!st && hipMemsetAsync(b->d_p0, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
     if (!st && hipMemsetAsync(b->d_p, 0, sizeof(double) * 12 * n_subsets, e->stream) != hipSuccess) st = DIC_ERR_DEVICE;
     if (!st && hipMemsetAsync(b->d_res, 0, sizeof(dic_subset_result) * n_subsets, e->stream) != hipSuccess)

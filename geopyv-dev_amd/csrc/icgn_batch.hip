@@ -31,8 +31,9 @@ int launch_mpw(const BatchArgs& A, int blocks, size_t lds, hipStream_t stream) {
 }
 
 // LDS row stride: 72 floats when every window of the class fits it (rows
-// then sit at immediate offsets, and 72 = 8 mod 64 spreads an 8x8 tile of
-// pixels over all 64 banks), else the window's own width.
+// then sit at immediate offsets, and 72 = 8 mod 32 spreads an 8x4 block of
+// pixels, one half-wavefront, over all 32 banks), else the window's own
+// width.
 template <int M, int WPS>
 int launch_ms(const BatchArgs& A, int blocks, size_t lds, bool s72, hipStream_t stream) {
     return s72 ? launch_mpw<M, 12, WPS, 72>(A, blocks, lds, stream)
@@ -115,11 +116,8 @@ BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.h = (int)f->h;
     A.wmax = (double)f->w - 1e-9;
     A.hmax = (double)f->h - 1e-9;
-    A.coords = b->d_coords;
-    A.tpl_index = b->d_tpl_index;
-    A.tpl_start = b->d_tpl_start;
+    A.meta = reinterpret_cast<const batch::SubMeta*>(b->d_meta);
     A.tpl_xy = b->d_tpl_xy_eng;
-    A.tpl_ext = b->d_tpl_ext;
     A.p0 = b->d_p0;
     A.p_out = b->d_p;
     A.res = b->d_res;

@@ -63,6 +63,17 @@
 namespace geodic {
 namespace batch {
 
+// What a workgroup needs to start its subset, in one record so that it
+// arrives in one round trip: the centre, the template (its pixels
+// tpl_xy[st, st + n) and extents dxmin, dxmax, dymin, dymax).
+struct SubMeta {
+    double xc, yc;
+    int64_t st;
+    int32_t n, t;
+    int32_t ext[4];
+};
+static_assert(sizeof(SubMeta) == 48, "SubMeta layout");
+
 struct BatchArgs {
     const float* fC;
     const float* gC;
@@ -71,11 +82,8 @@ struct BatchArgs {
     int w, h;
     double wmax, hmax;
     int64_t n_sub;
-    const double* coords;
-    const int32_t* tpl_index;
-    const int64_t* tpl_start;
+    const SubMeta* meta;     // per subset
     const int32_t* tpl_xy;
-    const int32_t* tpl_ext;  // 4 per template: dxmin, dxmax, dymin, dymax
     const int32_t* perm;
     const double* p0;
     double* p_out;
@@ -642,6 +650,57 @@ __device__ __forceinline__ void chol_solve_neg_rd(const double* L, const double*
     for (int i = 0; i < M; ++i) x[i] = -x[i];
 }
 
+// H⁻¹ of the symmetric positive definite Hessian given as its upper
+// triangle, row by row (hu[(a, b)], a <= b), through the Cholesky factor
+// L Lᵀ = H (one reciprocal per column, every step in registers) and the
+// two substitutions per column of the identity, each column stored (by the
+// lanes with `store`) as soon as it is solved; false when a pivot is not
+// positive (Hi is then meaningless).  Every lane computes the same values.
+template <int M>
+__device__ __forceinline__ bool chol_inverse_regs(const double* hu, double* Hi, bool store) {
+    double L[M][M], rd[M];
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        double d = hu[j * M - j * (j - 1) / 2];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d = fma(-L[j][k], L[j][k], d);
+        ok = ok && d > 0.0;
+        const double l = sqrt(d);
+        rd[j] = 1.0 / l;
+#pragma unroll
+        for (int i = j + 1; i < M; ++i) {
+            double t = hu[j * M - j * (j - 1) / 2 + (i - j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t = fma(-L[i][k], L[j][k], t);
+            L[i][j] = t * rd[j];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        double y[M], x[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            double t = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < i; ++k) t = fma(-L[i][k], y[k], t);
+            y[i] = t * rd[i];
+        }
+#pragma unroll
+        for (int i = M - 1; i >= 0; --i) {
+            double t = y[i];
+#pragma unroll
+            for (int k = i + 1; k < M; ++k) t = fma(-L[k][i], x[k], t);
+            x[i] = t * rd[i];
+        }
+        if (store) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) Hi[i * M + c] = x[i];
+        }
+    }
+    return ok;
+}
+
 // LDS writes of some lanes made visible to the other lanes of the same
 // wavefront.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -714,8 +773,10 @@ __device__ __forceinline__ void chol_inverse_column(const double* L, const doubl
 }
 
 // Order-1 composition W(p)·W(dp)⁻¹ (subset.rs:499-528) with one reciprocal
-// of the determinant; false when W(dp) is singular.
+// of the determinant; false when W(dp) is singular.  No contraction, so
+// that every instantiation (whole batches and lists alike) rounds alike.
 __device__ __forceinline__ bool compose6_fast(const double* p, const double* dp, double* out) {
+#pragma clang fp contract(off)
     const double a = 1.0 + dp[2], b = dp[4], c = dp[0];
     const double d = dp[3], e = 1.0 + dp[5], f = dp[1];
     const double det = a * e - b * d;
@@ -842,21 +903,23 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     constexpr int NL = M * (M + 1) / 2;
     constexpr int K = Modes<M>::K;
     constexpr int NA = 3 + M;                          // per-iteration sums
-    constexpr int NH1 = (M <= 2) ? NL : 0;             // Hessian entries summed in the first setup pass
+    constexpr int NH1 = (M <= 6) ? NL : 0;             // Hessian entries summed in the first setup pass
+    constexpr bool kHinv = true;                       // the update is one dot product per lane with H⁻¹
     constexpr int V1 = 2 + 2 * M + NH1;                // first setup pass
-    constexpr int V2 = 4 + 4 * K;                      // second setup pass
+    constexpr int V2 = 2;                              // second setup pass: Σa, Σa²
+    constexpr int VF = 4 + 6 * K;                      // final pass
     GEODIC_DYN_LDS(float, lds);
     float* win = lds;
     OffXY* sXY = (OffXY*)(lds + A.win_cap);
-    __shared__ double sH[M * M];
-    __shared__ double sL[NL];
-    __shared__ double sRd[M];
+    __shared__ double sH[M == 12 ? M * M : 1];  // order 2: the Hessian, then its factor's workspace
+    __shared__ double sL[M == 12 ? NL : 1];
+    __shared__ double sRd[M == 12 ? M : 1];
     __shared__ double sAS[2 * M];               // A_j, S_j
-    __shared__ float sDg[4 + 4 * Modes<M>::K];  // mx, my, 1/sdx, 1/sdy, rs, 1/rv
+    __shared__ float sDg[2];                    // mean gradient mx, my
     __shared__ double sP[12];                   // the current warp (its only copy)
     __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
     __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
-    __shared__ double sHv[M == 12 ? M * M : 1];  // order 2: the Hessian's inverse
+    __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
     __shared__ double sG[3];  // g_m, Δg, norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[2];
@@ -872,23 +935,25 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     // skips the atomic.
     const int64_t idx = blockIdx.x;
     if (idx >= A.n_sub) return;
-    // a short list's warps sit in host memory beside its dispatch order:
-    // the lead wavefront issues its row's read now, with the order's, so
-    // that the first plan does not wait for a second round trip
-    constexpr bool kEarlyP0 = LIST && P <= 4;
+    // the lead wavefront issues the read of its initial warp as early as
+    // it can (a list's sits in host memory beside the dispatch order), so
+    // that the first plan does not wait for another round trip
     double p0v = 0.0;
-    if constexpr (kEarlyP0) {
+    if constexpr (LIST) {
         if (wave == 0 && lane < M) p0v = A.p0_list[12 * idx + lane];
     }
     const int64_t s = A.perm[idx];
-    const int t = A.tpl_index[s];
-    const int64_t st = A.tpl_start[t];
-    const int n = (int)(A.tpl_start[t + 1] - st);
+    if constexpr (!LIST) {
+        if (wave == 0 && lane < M) p0v = A.p0[12 * s + lane];
+    }
+    const SubMeta& md = A.meta[s];
+    const int64_t st = md.st;
+    const int n = md.n;
     const int32_t* __restrict__ offs = A.tpl_xy + st;
     int32_t ext[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ext[q] = A.tpl_ext[4 * t + q];
-    const double xc = A.coords[2 * s], yc = A.coords[2 * s + 1];
+    for (int q = 0; q < 4; ++q) ext[q] = md.ext[q];
+    const double xc = md.xc, yc = md.yc;
     // integer base and FP32 fraction of the centre
     const double bxd = floor(xc), byd = floor(yc);
     const bool centre_ok = bxd > -1.0e9 && bxd < 1.0e9 && byd > -1.0e9 && byd < 1.0e9;
@@ -1013,40 +1078,32 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 sAS[j] = w1[2 + j] - dlt * w1[2 + M + j];
                 sAS[M + j] = w1[2 + M + j];
             }
-            if constexpr (NH1 > 0) {
-                int q = 2 + 2 * M;
-                for (int aa = 0; aa < M; ++aa)
-                    for (int b2 = aa; b2 < M; ++b2) {
-                        sH[aa * M + b2] = w1[q];
-                        sH[b2 * M + aa] = w1[q];
-                        ++q;
-                    }
+        }
+        if constexpr (NH1 > 0) {
+            // orders 0 and 1: the lead wavefront factorises and inverts the
+            // Hessian (summed in the pass above) in registers, while the
+            // other wavefronts go on to the second pass
+            if (wave == 0) {
+                const bool ok = chol_inverse_regs<M>(w1 + 2 + 2 * M, sHv, lane == 0);
+                if (lane == 0) sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
             }
         }
-        if constexpr (M == 6) hessian_rows<M, P, WPS, 0, 6>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
         if constexpr (M == 12) {
             hessian_rows<M, P, WPS, 0, 4>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
             hessian_rows<M, P, WPS, 4, 12>(gxr, gyr, sXY, n, tid, lane, wave, sH, sRed);
         }
-        // the lead lane factorises (its status reaches every wavefront
-        // through the barriers of the next reduction) ...
+        // (the factorisation's status reaches every wavefront through the
+        // barriers of the next reduction)
         if constexpr (M == 12) {
-            // order 2: the lead wavefront factorises
+            // order 2: the lead wavefront factorises over its lanes, then
+            // H⁻¹ = L⁻ᵀL⁻¹ once, lane c solving for column c, so each
+            // iteration's update is one dot product per lane instead of two
+            // dependent substitutions
             __syncthreads();
             if (wave == 0) {
                 const bool ok = cholesky_lanes<M>(sH, sL, sRd, lane);
                 if (lane == 0) sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
             }
-        } else if (tid == 0) {
-            const bool ok = cholesky<M>(sH, sL);
-            sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
-            if (ok)
-                for (int i = 0; i < M; ++i) sRd[i] = 1.0 / sL[i * (i + 1) / 2 + i];
-        }
-        if constexpr (M == 12) {
-            // order 2: H⁻¹ = L⁻ᵀL⁻¹ once, lane c of the lead wavefront
-            // solving for column c, so each iteration's update is one dot
-            // product per lane instead of two dependent substitutions
             __syncthreads();
             if (wave == 0 && lane < M && sCtl[0] == 0) chol_inverse_column<M>(sL, sRd, lane, sHv);
         }
@@ -1054,12 +1111,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         if (wave == 0) {
             double p0[M];
 #pragma unroll
-            for (int k = 0; k < M; ++k) {
-                if constexpr (kEarlyP0)
-                    p0[k] = readlane_f64(p0v, k);
-                else
-                    p0[k] = uni(LIST ? A.p0_list[12 * idx + k] : A.p0[12 * s + k]);
-            }
+            for (int k = 0; k < M; ++k) p0[k] = readlane_f64(p0v, k);
             const Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
             if (lane == 0) {
                 sPlan = pl;
@@ -1067,11 +1119,9 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             }
         }
 
-        // ---- second pass: a = f - f_m in place, and the diagnostic's
-        // reference statistics (subset.rs:698-727, 786-818): gradient
-        // spreads, mode means and mode spreads ----
+        // ---- second pass: a = f - f_m in place, and its sums (the
+        // diagnostic's reference statistics ride on the final pass) ----
         const float d32 = (float)dlt;
-        const float mx = (float)(w1[2 + M] / dn), my = (float)(w1[3 + M] / dn);
         float v2[V2];
 #pragma unroll
         for (int j = 0; j < V2; ++j) v2[j] = 0.f;
@@ -1079,25 +1129,10 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         for (int k = 0; k < P; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
-                const float2 d = off_xy(sXY[i]);
                 const float a = am[k] - d32;
                 am[k] = a;
                 v2[0] += a;
                 v2[1] = fmaf(a, a, v2[1]);
-                const float ax = gxr[k] - mx, ay = gyr[k] - my;
-                const float ax2 = ax * ax, ay2 = ay * ay;
-                v2[2] += ax2;
-                v2[3] += ay2;
-                float mo[4];
-                modes32<M>(d.x, d.y, mo);
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    const float m2 = mo[q] * mo[q];
-                    v2[4 + q] = fmaf(ax, mo[q], v2[4 + q]);
-                    v2[4 + K + q] = fmaf(ay, mo[q], v2[4 + K + q]);
-                    v2[4 + 2 * K + q] = fmaf(ax2, m2, v2[4 + 2 * K + q]);
-                    v2[4 + 3 * K + q] = fmaf(ay2, m2, v2[4 + 3 * K + q]);
-                }
             }
         }
         double w2[V2];
@@ -1108,22 +1143,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         if (tid == 0) {
             R->delta_f = delta_f;
             R->sigma = sqrt(sq / dn);
-            const double sdx = sqrt(fmax(w2[2] / dn, 1e-18)), sdy = sqrt(fmax(w2[3] / dn, 1e-18));
-            sDg[0] = mx;
-            sDg[1] = my;
-            sDg[2] = (float)(1.0 / sdx);
-            sDg[3] = (float)(1.0 / sdy);
-#pragma unroll
-            for (int q = 0; q < K; ++q) {
-                // mean of z·mode, and its spread from Σ(z·mode)² - n·mean²
-                const double rsx = w2[4 + q] / (sdx * dn), rsy = w2[4 + K + q] / (sdy * dn);
-                const double vx = w2[4 + 2 * K + q] / (sdx * sdx * dn) - rsx * rsx;
-                const double vy = w2[4 + 3 * K + q] / (sdy * sdy * dn) - rsy * rsy;
-                sDg[4 + q] = (float)rsx;
-                sDg[4 + K + q] = (float)rsy;
-                sDg[4 + 2 * K + q] = (float)(1.0 / sqrt(fmax(vx, 1e-18)));
-                sDg[4 + 3 * K + q] = (float)(1.0 / sqrt(fmax(vy, 1e-18)));
-            }
+            sDg[0] = (float)(w1[2 + M] / dn);  // Σ sdi_0 = Σ ∂x f
+            sDg[1] = (float)(w1[3 + M] / dn);  // Σ sdi_1 = Σ ∂y f
         }
     }
     if (delta_f == 0.0) {
@@ -1248,8 +1269,15 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 double gz[M];
 #pragma unroll
                 for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
+                // lane j: component j of -H⁻¹∇ZNSSD, then broadcast
                 double dp[M];
-                chol_solve_neg_rd<M>(sL, sRd, gz, dp);
+                double acc = 0.0;
+                if (lane < M) {
+#pragma unroll
+                    for (int k = 0; k < M; ++k) acc = fma(sHv[lane * M + k], gz[k], acc);
+                }
+#pragma unroll
+                for (int k = 0; k < M; ++k) dp[k] = -readlane_f64(acc, k);
                 double p[M];
 #pragma unroll
                 for (int k = 0; k < M; ++k) p[k] = sP[k];
@@ -1323,14 +1351,14 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     const float rf = uni((float)(1.0 / sK[1]));
     const float rg = it_done ? uni((float)(1.0 / delta_g)) : __int_as_float(0x7fc00000);
     const float mg32 = it_done ? uni((float)(g_m - sK[0])) : -f_m32;
-    float fin[2 + 2 * K];  // Σe², Σe, Σ (z·mode - mean)·e
+    // Σe², Σe, then per gradient axis (∇f less its mean: c) Σc², Σ c·mode,
+    // Σ (c·mode)², Σ c·mode·e: the diagnostic's gradient spreads, mode means
+    // and mode spreads (subset.rs:698-727, 786-818) and its projections
+    float fin[VF];
 #pragma unroll
-    for (int j = 0; j < 2 + 2 * K; ++j) fin[j] = 0.f;
+    for (int j = 0; j < VF; ++j) fin[j] = 0.f;
     {
-        const float mx = sDg[0], my = sDg[1], rsdx = sDg[2], rsdy = sDg[3];
-        float rs[2 * K];
-#pragma unroll
-        for (int q = 0; q < 2 * K; ++q) rs[q] = sDg[4 + q];
+        const float mx = sDg[0], my = sDg[1];
         const Window W = plan_window(pl);
         const bool fast = it_done && pl.fast;
         const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
@@ -1365,30 +1393,46 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 const float e = fmaf(am[k], rf, -(gp - mg32) * rg);
                 fin[0] = fmaf(e, e, fin[0]);
                 fin[1] += e;
-                const float zx = (gxr[k] - mx) * rsdx, zy = (gyr[k] - my) * rsdy;
+                const float cx = gxr[k] - mx, cy = gyr[k] - my;
+                fin[2] = fmaf(cx, cx, fin[2]);
+                fin[3] = fmaf(cy, cy, fin[3]);
                 float mo[4];
                 modes32<M>(d.x, d.y, mo);
 #pragma unroll
                 for (int qq = 0; qq < K; ++qq) {
-                    fin[2 + qq] = fmaf(fmaf(zx, mo[qq], -rs[qq]), e, fin[2 + qq]);
-                    fin[2 + K + qq] = fmaf(fmaf(zy, mo[qq], -rs[K + qq]), e, fin[2 + K + qq]);
+                    // explicit FMAs throughout: every instantiation rounds alike
+                    const float xm = cx * mo[qq], ym = cy * mo[qq];
+                    fin[4 + qq] = fmaf(cx, mo[qq], fin[4 + qq]);
+                    fin[4 + K + qq] = fmaf(cy, mo[qq], fin[4 + K + qq]);
+                    fin[4 + 2 * K + qq] = fmaf(xm, xm, fin[4 + 2 * K + qq]);
+                    fin[4 + 3 * K + qq] = fmaf(ym, ym, fin[4 + 3 * K + qq]);
+                    fin[4 + 4 * K + qq] = fmaf(xm, e, fin[4 + 4 * K + qq]);
+                    fin[4 + 5 * K + qq] = fmaf(ym, e, fin[4 + 5 * K + qq]);
                 }
             }
         }
     }
-    double fs[2 + 2 * K];
-    block_sum32<WPS, 2 + 2 * K>(fin, fs, sRed, lane, wave);
+    double fs[VF];
+    block_sum32<WPS, VF>(fin, fs, sRed, lane, wave);
     if (tid == 0) {
         const double znssd = it_done ? fs[0] : 4.0;
         const double zncc = it_done ? 1.0 - znssd / 2.0 : 0.0;
-        // Σ sc·(e - ē) with sc = (z·mode - mean)/spread; Σ sc = 0, so the
-        // centring of e drops out
+        // with z = c / sd (sd the gradient's spread), the standardised mode
+        // sc = (z·mode - rs) / spread (rs, spread: mean and spread of
+        // z·mode); Σ sc·(e - ē) = (Σ c·mode·e / sd - rs Σe) / spread, as
+        // Σ sc = 0 the centring of e drops out
         double eu = 0.0, ev = 0.0;
-        for (int qq = 0; qq < K; ++qq) {
-            const double a = (double)sDg[4 + 2 * K + qq] * fs[2 + qq] / dn;
-            const double b = (double)sDg[4 + 3 * K + qq] * fs[2 + K + qq] / dn;
-            eu += a * a;
-            ev += b * b;
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax) {
+            const double sd = sqrt(fmax(fs[2 + ax] / dn, 1e-18));
+            double ss = 0.0;
+            for (int qq = 0; qq < K; ++qq) {
+                const double rs = fs[4 + ax * K + qq] / (sd * dn);
+                const double var = fs[4 + (2 + ax) * K + qq] / (sd * sd * dn) - rs * rs;
+                const double pr = (fs[4 + (4 + ax) * K + qq] / sd - rs * fs[1]) / (sqrt(fmax(var, 1e-18)) * dn);
+                ss += pr * pr;
+            }
+            (ax == 0 ? eu : ev) = ss;
         }
         for (int k = 0; k < 12; ++k) {
             const double pk = k < M ? sP[k] : 0.0;

@@ -123,11 +123,17 @@ int main(int argc, char** argv) {
     A.wmax = W - 1e-9;
     A.hmax = H - 1e-9;
     A.n_sub = n;
-    A.coords = coords.data();
-    A.tpl_index = tpl_index.data();
-    A.tpl_start = tpl_start.data();
+    std::vector<geodic::batch::SubMeta> meta(n);
+    for (int s = 0; s < n; ++s) {
+        meta[s].xc = coords[2 * s];
+        meta[s].yc = coords[2 * s + 1];
+        meta[s].st = 0;
+        meta[s].n = npx;
+        meta[s].t = 0;
+        for (int q = 0; q < 4; ++q) meta[s].ext[q] = ext[q];
+    }
+    A.meta = meta.data();
     A.tpl_xy = packed.data();
-    A.tpl_ext = ext;
     A.perm = perm.data();
     A.p0 = p0.data();
     A.p_out = pout.data();
