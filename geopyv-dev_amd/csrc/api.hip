@@ -792,6 +792,63 @@ int dic_batch_subset_arrays(dic_batch* db, int64_t s, double* f, double* grad_f)
     return DIC_OK;
 }
 
+int dic_batch_residual_map(dic_batch* db, const dic_image* f, const dic_image* g, const int32_t* idx, int64_t count,
+                           const double* p, double* out) {
+    CHECK_ARG(db && f && g, "dic_batch_residual_map: NULL argument");
+    CHECK_ARG(count >= 0, "invalid input: negative list length");
+    CHECK_ARG(count == 0 || (idx && out), "dic_batch_residual_map: NULL array");
+    Batch* b = &db->b;
+    Engine* e = b->eng;
+    CHECK_ARG(f->im.eng->device == e->device && g->im.eng->device == e->device,
+              "invalid input: images live on another device");
+    CHECK_ARG(f->im.h == g->im.h && f->im.w == g->im.w && f->im.border == g->im.border,
+              "invalid input: reference and target images differ in shape or border");
+    std::vector<int64_t> off(count + 1, 0);
+    for (int64_t k = 0; k < count; ++k) {
+        CHECK_ARG(idx[k] >= 0 && idx[k] < b->n, "invalid input: subset index out of range");
+        const int t = b->h_tpl_index[idx[k]];
+        off[k + 1] = off[k] + (b->h_tpl_start[t + 1] - b->h_tpl_start[t]);
+    }
+    DIC_HIP_TRY(hipSetDevice(e->device));
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!p && b->last_precision < 0) {
+        set_error("not solved: residual_map: the batch has no stored solution");
+        return DIC_ERR_NOT_SOLVED;
+    }
+    if (count == 0) return DIC_OK;
+    // one device block: list, offsets, warps (when given), residuals
+    const size_t list_b = ((size_t)count * 4 + 7) & ~(size_t)7;
+    const size_t off_b = sizeof(int64_t) * (size_t)count;
+    const size_t p_b = p ? sizeof(double) * 12 * (size_t)count : 0;
+    const size_t out_b = sizeof(double) * (size_t)off[count];
+    char* d = nullptr;
+    int st = dev_alloc(e, (void**)&d, list_b + off_b + p_b + out_b);
+    if (st) return st;
+    auto fail = [&](const char* what) {
+        set_error(std::string("dic_batch_residual_map: ") + what);
+        (void)hipStreamSynchronize(e->stream);
+        dev_free(e, d, list_b + off_b + p_b + out_b);
+        return DIC_ERR_DEVICE;
+    };
+    if (hipMemcpyAsync(d, idx, sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+        hipMemcpyAsync(d + list_b, off.data(), off_b, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+        (p && hipMemcpyAsync(d + list_b + off_b, p, p_b, hipMemcpyHostToDevice, e->stream) != hipSuccess))
+        return fail("upload failed");
+    st = launch_residual(b, &f->im, &g->im, reinterpret_cast<const int32_t*>(d), count,
+                         p ? reinterpret_cast<const double*>(d + list_b + off_b) : b->d_p, p == nullptr,
+                         reinterpret_cast<const int64_t*>(d + list_b), reinterpret_cast<double*>(d + list_b + off_b + p_b));
+    if (st) {
+        (void)hipStreamSynchronize(e->stream);
+        dev_free(e, d, list_b + off_b + p_b + out_b);
+        return st;
+    }
+    if (hipMemcpyAsync(out, d + list_b + off_b + p_b, out_b, hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+        hipStreamSynchronize(e->stream) != hipSuccess)
+        return fail("download failed");
+    dev_free(e, d, list_b + off_b + p_b + out_b);
+    return DIC_OK;
+}
+
 int dic_batch_device_p(dic_batch* db, const double** out) {
     CHECK_ARG(db && out, "dic_batch_device_p: NULL argument");
     *out = db->b.d_p;

@@ -613,6 +613,125 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
 }
 
 // ---------------------------------------------------------------------------
+// Subset::residual_map (subset.rs:1350-1392, deformable texture): the
+// per-pixel zero-normalised residual e_i = (f_i - f_m)/Δf - (g_i - g_m)/Δg
+// at warp p, FP64 with the reference's association, one 64-lane wavefront
+// per listed subset.  The reference intensities pass through the output
+// row; g is evaluated once per use (sums, spread, output) rather than kept.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ResidualArgs {
+    const double* fC;
+    const double* gC;
+    int64_t wp;
+    int border;
+    double wmax, hmax;
+    int64_t count;
+    const int32_t* list;
+    const double* coords;
+    const int32_t* tpl_index;
+    const int64_t* tpl_start;
+    const int32_t* tpl_xy;
+    const double* p;        // warps: 12 per listed subset, or per subset (p_by_subset)
+    int p_by_subset;
+    const int64_t* out_off;  // start of each listed subset's row in out
+    double* out;
+};
+
+template <int M>
+__global__ void __launch_bounds__(64) residual_kernel(ResidualArgs A) {
+#pragma clang fp contract(off)
+    const int64_t k = blockIdx.x;
+    if (k >= A.count) return;
+    const int64_t s = A.list[k];
+    const int lane = threadIdx.x;
+    const int t = A.tpl_index[s];
+    const int64_t st = A.tpl_start[t];
+    const int64_t n = A.tpl_start[t + 1] - st;
+    const int32_t* offs = A.tpl_xy + st;
+    const double xc = A.coords[2 * s], yc = A.coords[2 * s + 1];
+    const double dn = (double)n;
+    double* e = A.out + A.out_off[k];
+    const double* pr = A.p + 12 * (A.p_by_subset ? s : k);
+    double p[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) p[q] = (q < M) ? pr[q] : 0.0;
+    auto pos = [&](int64_t i, double& x, double& y) {
+        const int32_t o = offs[i];
+        x = (double)unpack_dx(o) + xc;
+        y = (double)unpack_dy(o) + yc;
+    };
+    auto g_at = [&](int64_t i) {
+        double x, y, wx, wy;
+        pos(i, x, y);
+        warp_point<M>(p, x, y, xc, yc, wx, wy);
+        return eval_f64(A.gC, A.wp, A.border, A.wmax, A.hmax, wx, wy);
+    };
+    // f, f_m and Δf (subset.rs:877-886)
+    double part = 0.0;
+    for (int64_t i = lane; i < n; i += 64) {
+        double x, y;
+        pos(i, x, y);
+        const double v = eval_f64(A.fC, A.wp, A.border, A.wmax, A.hmax, x, y);
+        e[i] = v;
+        part += v;
+    }
+    const double f_m = wave_sum_u(part) / dn;
+    part = 0.0;
+    for (int64_t i = lane; i < n; i += 64) {
+        const double d = e[i] - f_m;
+        part += d * d;
+    }
+    const double delta_f = sqrt(wave_sum_u(part));
+    // g at the warped positions, g_m and Δg (subset.rs:1385-1388)
+    part = 0.0;
+    for (int64_t i = lane; i < n; i += 64) part += g_at(i);
+    const double g_m = wave_sum_u(part) / dn;
+    part = 0.0;
+    for (int64_t i = lane; i < n; i += 64) {
+        const double d = g_at(i) - g_m;
+        part += d * d;
+    }
+    const double delta_g = sqrt(wave_sum_u(part));
+    for (int64_t i = lane; i < n; i += 64) e[i] = (e[i] - f_m) / delta_f - (g_at(i) - g_m) / delta_g;
+}
+
+}  // namespace
+
+int launch_residual(Batch* b, const Image* f, const Image* g, const int32_t* d_list, int64_t count, const double* d_p,
+                    bool p_by_subset, const int64_t* d_off, double* d_out) {
+    Engine* e = b->eng;
+    ResidualArgs A;
+    A.fC = f->coef64;
+    A.gC = g->coef64;
+    A.wp = f->wp;
+    A.border = f->border;
+    A.wmax = (double)f->w - 1e-9;
+    A.hmax = (double)f->h - 1e-9;
+    A.count = count;
+    A.list = d_list;
+    A.coords = b->d_coords;
+    A.tpl_index = b->d_tpl_index;
+    A.tpl_start = b->d_tpl_start;
+    A.tpl_xy = b->d_tpl_xy;
+    A.p = d_p;
+    A.p_by_subset = p_by_subset ? 1 : 0;
+    A.out_off = d_off;
+    A.out = d_out;
+    if (count == 0) return DIC_OK;
+    const dim3 grid((unsigned)count), block(64);
+    if (b->max_order == 0)
+        residual_kernel<2><<<grid, block, 0, e->stream>>>(A);
+    else if (b->max_order == 1)
+        residual_kernel<6><<<grid, block, 0, e->stream>>>(A);
+    else
+        residual_kernel<12><<<grid, block, 0, e->stream>>>(A);
+    DIC_HIP_TRY(hipGetLastError());
+    return DIC_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Fixed scratch of dic_batch_solve_noalloc: a file-scope device array that
 // every GPU the library runs on holds for the life of the process.  In host
 // code `scratch` names a host-side stand-in, not a device address, so host

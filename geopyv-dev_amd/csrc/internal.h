@@ -146,6 +146,11 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
                       const int32_t* list = nullptr, int64_t count = 0, const double* p0_list = nullptr,
                       dic_subset_result* res_list = nullptr, double* ext_scratch = nullptr, size_t ext_doubles = 0);
+// Subset::residual_map for the `count` listed subsets (device list), at the
+// warps d_p (12 per listed subset, or per subset of the batch when
+// p_by_subset), written to d_out from d_off[k] for listed subset k.
+int launch_residual(Batch* b, const Image* f, const Image* g, const int32_t* d_list, int64_t count, const double* d_p,
+                    bool p_by_subset, const int64_t* d_off, double* d_out);
 // per-pixel doubles of the plain kernel's scratch: f, grad f, g (+ grad g for FAGN)
 inline int plain_arrays(int method) { return method == DIC_METHOD_FAGN ? 6 : 4; }
 // The library's fixed device scratch for dic_batch_solve_noalloc: its

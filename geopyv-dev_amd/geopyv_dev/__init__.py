@@ -388,7 +388,9 @@ class Subset:
         (subset.rs:1350-1392)."""
         if self._result is None:
             raise RuntimeError("invalid input: residual_map: subset has not been solved")
-        raise NotImplementedError("residual_map is not exposed by the engine yet")
+        p = np.zeros((1, 12))
+        p[0, :len(self._result["p"])] = self._result["p"]
+        return self._batch.residual_map(self._f_img._dev, self._g_img._dev, [0], p)[0]
 
     # always-available getters
     coord = property(lambda self: list(self._coord))

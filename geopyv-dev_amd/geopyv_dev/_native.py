@@ -45,7 +45,7 @@ EXPORTED = [
     "dic_batch_set_schedule", "dic_batch_autotune", "dic_batch_device_p", "dic_batch_destroy",
     "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
     "dic_frame_graph_destroy", "dic_batch_solve_list", "dic_mesh_solve", "dic_mesh_element_strains",
-    "dic_image_grey", "dic_batch_solve_noalloc", "dic_batch_get_schedule",
+    "dic_image_grey", "dic_batch_solve_noalloc", "dic_batch_get_schedule", "dic_batch_residual_map",
 ]
 
 
@@ -194,6 +194,7 @@ def load():
         L.dic_batch_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.dic_batch_autotune.argtypes = [_vp, _vp, _vp, ctypes.POINTER(SolveParams)]
         L.dic_batch_get_schedule.argtypes = [_vp, _vp, _vp]
+        L.dic_batch_residual_map.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp]
         L.dic_image_stage.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
         L.dic_batch_reset_warps.argtypes = [_vp, _vp, ctypes.c_int]
         L.dic_frame_graph_create.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(SolveParams),
@@ -474,6 +475,19 @@ class Batch:
         g = np.empty((n, 2))
         check(load().dic_batch_subset_arrays(self.handle, int(s), ptr(f), ptr(g)))
         return f, g
+
+    def residual_map(self, f_img, g_img, idx, p=None):
+        """Subset::residual_map of the listed subsets at warps p (len(idx) x
+        12), or at the stored final warps; a list of per-pixel arrays."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32).ravel()
+        sizes = [int(self.tpl_start[self.tpl_index[s] + 1] - self.tpl_start[self.tpl_index[s]]) for s in idx]
+        out = np.zeros(int(sum(sizes)))
+        pp = None
+        if p is not None:
+            pp = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(len(idx), 12))
+        check(load().dic_batch_residual_map(self.handle, f_img.handle, g_img.handle, ptr(idx), len(idx),
+                                            None if pp is None else ptr(pp), ptr(out)))
+        return np.split(out, np.cumsum(sizes)[:-1])
 
     def device_p(self):
         p = _vp()

@@ -239,6 +239,15 @@ int dic_frame_graph_create(dic_batch* b, const dic_image* f_img, dic_image* g_im
 /* Enqueues one frame on the engine's stream (dic_engine_sync waits for it). */
 int dic_frame_graph_launch(dic_frame_graph* g);
 void dic_frame_graph_destroy(dic_frame_graph* g);
+/* Subset::residual_map (subset.rs:1350-1392): for each listed subset the
+ * per-pixel zero-normalised residual e_i = (f_i - f_m)/Δf - (g_i - g_m)/Δg
+ * at warp p (count x 12 doubles, host), or at the batch's stored final warps
+ * when p is NULL (DIC_ERR_NOT_SOLVED if the batch was never solved),
+ * evaluated in FP64 on the device.  out (host) receives the subsets' rows
+ * one after another in list order, n_px doubles each in the template's
+ * pixel order.  Waits for the device. */
+int dic_batch_residual_map(dic_batch* b, const dic_image* f_img, const dic_image* g_img, const int32_t* idx,
+                           int64_t count, const double* p, double* out);
 /* Final warps as a device pointer (n_subsets x 12 doubles) for chaining
  * batches without a host round trip. */
 int dic_batch_device_p(dic_batch* b, const double** out);

@@ -927,6 +927,28 @@ int orc_subset_solve(const double* fcoef, const double* gcoef, int h, int w, int
     return st;
 }
 
+// Subset::residual_map at warp p (subset.rs:1350-1392, deformable texture,
+// residual_textured 1367-1392): Subset::new, the warped target intensities,
+// g_m = Σg / n (ndarray's sum), Δg = √Σ(g - g_m)², then
+// e_i = (f_i - f_m)/Δf - (g_i - g_m)/Δg.  p holds p_len entries of `order`.
+int orc_residual_map(const double* fcoef, const double* gcoef, int h, int w, int border, const double* coord,
+                     const double* offsets, int64_t n, int order, const double* p, double* out) {
+    Img fim{fcoef, h, w, border}, gim{gcoef, h, w, border};
+    SubsetRef S;
+    const int st = subset_new(fim, coord, offsets, n, order, S);
+    if (st) return st;
+    const int m = p_len(order);
+    std::vector<double> gc(2 * n), g(n);
+    orc_apply_warp(S.coord, p, m, S.f_coords.data(), n, gc.data());
+    for (int64_t i = 0; i < n; ++i) g[i] = img_eval(gim, gc[2 * i], gc[2 * i + 1]);
+    const double g_m = unrolled_sum(g.data(), n) / (double)n;
+    double dg = 0.0;
+    for (int64_t i = 0; i < n; ++i) { const double t = g[i] - g_m; dg += t * t; }
+    const double delta_g = std::sqrt(dg);
+    for (int64_t i = 0; i < n; ++i) out[i] = (S.f[i] - S.f_m) / S.delta_f - (g[i] - g_m) / delta_g;
+    return ORC_OK;
+}
+
 // Batch of subsets sharing one offset table per template.  The CPU baseline:
 // one subset per task, as mesh.rs:1224-1233 parallelises a colour round.
 // tpl_start has n_tpl+1 entries into tpl_xy (pairs of doubles).

@@ -38,6 +38,11 @@ compose_icgn           golden vector (subset.rs:1790-1810); identity, order
 convergence_norm       closed forms (subset.rs:1425-1456)
 apply_warp / sdi       every-slot quad warp (subset.rs:2221-2259)
 zn_slice / eta         hand-computed values (subset.rs:1460-1616)
+residual_map           zero on identical images at p = 0; Σe = 0; Σe² equal
+                       to the ZNSSD a one-iteration solve started at p
+                       reports (delta_p_and_znssd, an independent pass);
+                       translated pair: Σe² at the solution below Σe² at
+                       p = 0
 Subset::new statistics f_m, delta_f, SSSIG, sigma against NumPy on the
 (f_m, delta_f, sssig,  oracle's own f / grad_f; a hand-computed four-pixel
 sigma)                 case on a linear ramp (f_m 332.5, delta_f sqrt 13,
@@ -339,6 +344,25 @@ def subset_solve(fcoef, gcoef, shape, border, coord, offsets, order, p0=None, me
     }
     if want_arrays:
         out.update(f=f_out, grad_f=g_out, f_coords=fc_out)
+    return out
+
+
+def residual_map(fcoef, gcoef, shape, border, coord, offsets, order, p):
+    """Subset::residual_map at warp p (subset.rs:1350-1392): the per-pixel
+    zero-normalised residual e_i = zn(f_i) - zn(g_i)."""
+    fcoef = _c64(fcoef)
+    gcoef = _c64(gcoef)
+    coord = _c64(coord)
+    off = _c64(offsets)
+    m = {0: 2, 1: 6, 2: 12}[order]
+    pv = np.zeros(12)
+    v = np.asarray(p, dtype=np.float64).ravel()[:m]
+    pv[:len(v)] = v
+    out = np.zeros(len(off))
+    st = lib().orc_residual_map(_d(fcoef), _d(gcoef), shape[0], shape[1], border, _d(coord), _d(off),
+                                ctypes.c_int64(len(off)), order, _d(pv), _d(out))
+    if st:
+        raise RuntimeError(f"residual_map: status {st}")
     return out
 
 

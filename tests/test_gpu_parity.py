@@ -306,3 +306,50 @@ def test_autotuned_schedule_solves_within_the_bar(band_small):
         runs.append((sched, r))
     if runs[0][0] == runs[1][0]:
         assert runs[0][1].tobytes() == runs[1][1].tobytes()
+
+
+# ---- Subset::residual_map (subset.rs:1350-1392) on the device, FP64 ----
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_residual_map_matches_oracle(pair0, order):
+    ref, tar = pair0["ref"], pair0["tar"]
+    f = gp.Image(image_gs=ref, border=20)
+    g = gp.Image(image_gs=tar, border=20)
+    m = gp.Mask(mask_type="local", shape="circle", size=25)
+    s = gp.Subset([256.0, 256.0], m, f, g, subset_order=order)
+    with pytest.raises(RuntimeError):
+        s.residual_map()
+    s.solve(max_norm=1e-5)
+    e = s.residual_map()
+    eo = O.residual_map(pair0["Cf"], pair0["Cg"], ref.shape, 20, [256.0, 256.0], m.coords, order, s.p)
+    print(f"order {order}: max |de| {np.abs(e - eo).max():.2e}, sum e {e.sum():.2e}, sum e^2 {np.sum(e * e):.4e}, "
+          f"c_znssd {s.c_znssd:.4e}")
+    assert e.shape == (s.n_px,)
+    assert np.abs(e - eo).max() < 1e-10
+    assert abs(e.sum()) < 1e-9
+    # converged: the last update moved the warp by less than max_norm, so
+    # the residual at the stored warp is the last iteration's to a few 1e-6
+    assert abs(np.sum(e * e) - s.c_znssd) < 1e-4 * max(s.c_znssd, 1e-3)
+
+
+def test_batch_residual_map_stored_and_given_warps(band_small):
+    b = band_small
+    f = gp.Image(image_gs=b["ref"], border=20)
+    g = gp.Image(image_gs=b["tar"], border=20)
+    index, start, xy = gp.templates_from_masks([mm.coords for mm in b["local"]])
+    bt = _native.Batch(gp._engine(), b["coords"], 2, index, start, xy)
+    bt.set_p0(b["p0"])
+    with pytest.raises(_native.GeodicError) as ei:
+        bt.residual_map(f._dev, g._dev, [0])
+    assert ei.value.status == 8
+    bt.solve(f._dev, g._dev, precision=_native.PREC_COMP_F32, max_norm=1e-5)
+    r = bt.results()
+    pick = [0, 7, 42, 150, 299, 3]
+    maps = bt.residual_map(f._dev, g._dev, pick)
+    given = bt.residual_map(f._dev, g._dev, pick, p=b["p0"][pick])
+    Cf, Cg = O.get_c(b["ref"], 20), O.get_c(b["tar"], 20)
+    for k, s in enumerate(pick):
+        off = b["local"][s].coords
+        eo = O.residual_map(Cf, Cg, b["ref"].shape, 20, b["coords"][s], off, 2, r["p"][s])
+        assert np.abs(maps[k] - eo).max() < 1e-10
+        eo0 = O.residual_map(Cf, Cg, b["ref"].shape, 20, b["coords"][s], off, 2, b["p0"][s])
+        assert np.abs(given[k] - eo0).max() < 1e-10

@@ -483,3 +483,35 @@ def test_subset_new_statistics_four_pixel_known_answer():
     assert abs(r["delta_f"] - np.sqrt(13.0)) < 1e-9
     assert abs(r["sigma"] - np.sqrt(13.0 / 4.0)) < 1e-9
     assert abs(r["sssig"] - 26.0) < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# Subset::residual_map (subset.rs:1350-1392)
+# ---------------------------------------------------------------------------
+def test_residual_map_zero_for_identical_images(speckle_pair):
+    f, _, Cf, _ = speckle_pair
+    off, _ = O.mask_local("circle", 15)
+    e = O.residual_map(Cf, Cf, f.shape, 20, [64.0, 64.0], off, 1, np.zeros(6))
+    assert len(e) == len(off) and np.all(e == 0.0)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_residual_map_sums_match_the_solver(speckle_pair, order):
+    # Σe = 0 (both terms are zero-mean) and Σe² is the ZNSSD that a
+    # one-iteration solve started at p computes in its own pass
+    f, g, Cf, Cg = speckle_pair
+    off, _ = O.mask_local("circle", 15)
+    m = {0: 2, 1: 6, 2: 12}[order]
+    p = np.zeros(m)
+    p[:2] = [0.3, -0.45]
+    if order >= 1:
+        p[2:6] = [0.004, -0.002, 0.001, 0.003]
+    e = O.residual_map(Cf, Cg, f.shape, 20, [63.4, 64.7], off, order, p)
+    r = O.subset_solve(Cf, Cg, f.shape, 20, [63.4, 64.7], off, order, p0=p, max_iterations=1)
+    assert abs(np.sum(e)) < 1e-12
+    assert abs(np.sum(e * e) - r["c_znssd"]) < 1e-12 * max(1.0, r["c_znssd"])
+    # at the converged warp the residual is smaller than at p = 0
+    rs = O.subset_solve(Cf, Cg, f.shape, 20, [63.4, 64.7], off, order, max_norm=1e-8)
+    e0 = O.residual_map(Cf, Cg, f.shape, 20, [63.4, 64.7], off, order, np.zeros(m))
+    es = O.residual_map(Cf, Cg, f.shape, 20, [63.4, 64.7], off, order, rs["p"])
+    assert np.sum(es * es) < 0.5 * np.sum(e0 * e0)
