@@ -16,7 +16,8 @@ import numpy as np
 from . import _native
 from ._native import GeodicError
 
-__all__ = ["Image", "Mask", "Subset", "Mesh", "Sequence", "structured_mesh", "quadratic_mesh", "solve_subsets", "GeodicError"]
+__all__ = ["Image", "Mask", "Subset", "Mesh", "Sequence", "structured_mesh", "quadratic_mesh", "solve_subsets",
+           "GeodicError", "save", "load"]
 
 
 def _engine():
@@ -566,3 +567,21 @@ def solve_sequence(frames, coords, local_mask, subset_order=1, reference_every=2
 
 from .mesh import Mesh, quadratic_mesh, structured_mesh  # noqa: E402  (uses Image, Mask and the solver helpers above)
 from .sequence import Sequence  # noqa: E402
+
+
+def save(path, obj):
+    """Serialise a solved mesh (a Mesh or its MeshSolution) to a `.pyv` file
+    (geopyv_dev/__init__.py:81-86 of the reference; other solution types
+    are outside the correlation engine)."""
+    from .pyv import MeshSolution, save_mesh
+    if isinstance(obj, Mesh):
+        obj = obj.solution()
+    if not isinstance(obj, MeshSolution):
+        raise TypeError("save: only mesh solutions are written by this package")
+    save_mesh(path, obj)
+
+
+def load(path):
+    """Load a `.pyv` mesh solution (MeshSolution record)."""
+    from .pyv import load_mesh
+    return load_mesh(path)

@@ -275,7 +275,37 @@ class Mesh:
             subset_order=int(subset_order), iterations=r["iterations"].astype(np.uint32), norms=r["norms"],
             quality_ok=r["quality_ok"], template_shape=local_mask.shape,
             template_sizes=np.full(len(self._nodes), local_mask.size, dtype=np.uint32),
-            stats=dict(node_solves=r["n_node_solves"], lists=r["n_lists"]))
+            stats=dict(node_solves=r["n_node_solves"], lists=r["n_lists"]),
+            solve_config=dict(max_norm=float(max_norm), max_iterations=int(max_iterations),
+                              subset_order=int(subset_order), tolerance=float(tolerance), method=method,
+                              override_active=bool(override_active), preconditioning=pre,
+                              batch_factor=batch_factor, root_rel_eps=float(lrg.get("root_rel_eps", 0.02)),
+                              max_workers=lrg.get("max_workers")),
+            seed=dict(coord=[float(seed_coord[0]), float(seed_coord[1])], warp=np.resize(warp, m).copy(),
+                      tolerance=float(seed_tolerance)))
+
+    def solution(self):
+        """The solved mesh as the MeshSolution record of a `.pyv` file
+        (mesh.rs:54-113)."""
+        from .pyv import MeshSolution, boundary_nodes
+        if self._solution is None:
+            raise RuntimeError("Mesh has not been solved; call solve() first")
+        s = self._solution
+        t = self._time
+        return MeshSolution(
+            nodes=self._nodes, elements=self._elements, boundary=boundary_nodes(self._elements, self._mesh_order),
+            exclusions=[], centroids=s["centroids"], areas=s["areas"], warps=s["warps"],
+            displacements=s["displacements"], c_zncc=s["c_zncc"], p=s["p"], seed_node=s["seed_node"],
+            mesh_order=self._mesh_order, subset_order=s["subset_order"], iterations=s["iterations"],
+            norms=s["norms"], f_img_path=self._f_img.filepath or "", g_img_path=self._g_img.filepath or "",
+            solve_config=s["solve_config"], seed=s["seed"], template_shape=s["template_shape"],
+            template_sizes=s["template_sizes"], time=None if t is None else (float(t[0]), float(t[1])))
+
+    def save(self, path):
+        """Save the solved mesh to a `.pyv` file (GPYV, version 0x17,
+        bincode v2, zstd level 3: io.rs:2796-2816)."""
+        from .pyv import save_mesh
+        save_mesh(path, self.solution())
 
     def __repr__(self):
         state = "solved" if self.solved else "unsolved"

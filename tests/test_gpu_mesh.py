@@ -315,3 +315,24 @@ def test_mesh_class_cancel_stores_nothing(speckle):
     m.solve(gp.Mask("local", shape="circle", size=20), seed_coord=[256.0, 256.0], tolerance=0.5, seed_tolerance=0.5,
             progress=obs)
     assert m.solved and obs.started == [len(c["nodes"])] and obs.done >= len(c["nodes"])
+
+
+def test_solved_mesh_saves_as_pyv(band, tmp_path):
+    # a solved Mesh written as .pyv (GPYV, 0x17, bincode v2, zstd) loads back
+    # with every array of the solution unchanged
+    import os
+    f = gp.Image(image_gs=band["ref"], border=15)
+    g = gp.Image(image_gs=band["tar"], border=15)
+    m = gp.Mesh(f_img=f, g_img=g, nodes=band["nodes"], elements=band["el"])
+    m.solve(gp.Mask("local", shape="circle", size=6), seed_coord=[50.0, 30.0], seed_warp=[0.0] * 6,
+            tolerance=0.5, seed_tolerance=0.5, override_active=True, solver_options={"preconditioning": "layer-RG"})
+    path = os.path.join(tmp_path, "band.pyv")
+    m.save(path)
+    raw = open(path, "rb").read()
+    assert raw[:5] == b"GPYV\x17"
+    s = gp.load(path)
+    assert np.array_equal(s.displacements, m.displacements) and np.array_equal(s.p, m.p)
+    assert np.array_equal(s.c_zncc, m.c_zncc) and np.array_equal(s.iterations, m.iterations)
+    assert np.array_equal(s.warps, m.warps) and np.array_equal(s.nodes, m.nodes)
+    assert np.array_equal(s.elements, m.elements) and s.seed_node == m.seed_node
+    assert s.solve_config["preconditioning"] == "layer-RG" and s.template_shape == "circle"
