@@ -231,7 +231,8 @@ def run_sequence(args, torch, gp, _native, workloads):
     # the first nf frames of the 200-frame sequence (same slip per frame)
     frames, _ = workloads.sequence_frames(seed=3, shape=(2048, 2048), n_frames=nf, n_speckles=80000,
                                           slip_max=8.0 * (nf - 1) / (n_total - 1),
-                                          progress=lambda k: k % 10 == 9 and log(f"frame {k + 1} generated"))
+                                          progress=lambda k: k % 10 == 9 and log(f"frame {k + 1} generated"),
+                                          workers=min(16, host_cpu()[0]) if nf > 8 else 1)
     log(f"{nf} frames generated in {time.time() - t0:.1f}s")
     coords = workloads.grid_coords((2048, 2048), 11, 66, n_side=None)
     coords = coords[:30000]
@@ -462,9 +463,11 @@ def run_serving(args, gp, workloads):
                                   "4 slots, reference cache, HBM budget 64 GiB",
                       "jobs_per_rate": args.jobs},
            "rates": {}}
-    for lam in rates:
-        jobs, arrivals, t = [], [], 0.0
-        for j in range(args.jobs):
+    def job_stream(lam, n_jobs):
+        # jobs are made as they arrive, so a long stream never holds more
+        # than the queue
+        t = 0.0
+        for j in range(n_jobs):
             t += rng.exponential(1.0 / lam)
             shape = sizes[int(rng.integers(0, 3))]
             h, w = shape
@@ -476,21 +479,34 @@ def run_serving(args, gp, workloads):
             truth = field.warp(coords[:, 0], coords[:, 1])
             p0 = np.zeros((nsub, 12))
             p0[:, :2] = np.rint(truth[:, :2])
-            jobs.append(serving.Job(j, shape, ref, tar, coords, rng.integers(15, 41, nsub),
-                                    order=int(rng.integers(1, 3)), p0=p0))
-            arrivals.append(t)
+            yield t, serving.Job(j, shape, ref, tar, coords, rng.integers(15, 41, nsub), order=int(rng.integers(1, 3)),
+                                 p0=p0)
+
+    def summary(job, res, lat_ms):
+        return len(job.coords), int(np.sum(res["converged"])), lat_ms
+
+    for lam in rates:
         srv = serving.Server(n_slots=4, hbm_budget=64 << 30)
         tb = time.perf_counter()
-        res = srv.run(jobs, arrivals=arrivals)
+        last = [0.0]
+
+        def progress(done, submitted):
+            now = time.perf_counter()
+            if now - last[0] > 20.0:
+                last[0] = now
+                log(f"rate {lam}: {done} of {submitted} submitted jobs done, {now - tb:.0f}s")
+
+        res = srv.stream(job_stream(lam, args.jobs), summary, progress)
         wall = time.perf_counter() - tb
         srv.close()
-        lat = np.array([res[j.job_id][1] for j in jobs])
-        nsub_total = sum(len(j.coords) for j in jobs)
-        conv = np.mean(np.concatenate([res[j.job_id][0]["converged"] for j in jobs]))
-        out["rates"][str(lam)] = {"subsets_per_s": round(nsub_total / wall, 1), "jobs": len(jobs),
-                                  "wall_s": round(wall, 3), "latency_ms": {
+        lat = np.array([v[2] for v in res.values()])
+        nsub_total = sum(v[0] for v in res.values())
+        conv = sum(v[1] for v in res.values()) / max(nsub_total, 1)
+        out["rates"][str(lam)] = {"subsets_per_s": round(nsub_total / wall, 1), "jobs": len(res),
+                                  "subsets": int(nsub_total), "wall_s": round(wall, 3), "latency_ms": {
                                       "p50": round(float(np.percentile(lat, 50)), 3),
-                                      "p99": round(float(np.percentile(lat, 99)), 3)},
+                                      "p99": round(float(np.percentile(lat, 99)), 3),
+                                      "max": round(float(lat.max()), 3)},
                                   "converged_fraction": round(float(conv), 5)}
         log(f"rate {lam}: {out['rates'][str(lam)]}")
     out["value"] = out["rates"][str(rates[0])]["subsets_per_s"]

@@ -80,6 +80,7 @@ class Server:
         self.cv = threading.Condition()
         self.results = {}
         self.errors = {}
+        self.reduce = None  # job, results, latency_ms -> what results[] keeps (all of it when None)
         self.closed = False
         self.free_slots = list(range(n_slots))
         self.workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(n_slots)]
@@ -162,8 +163,10 @@ class Server:
             try:
                 res = self._run(job, self.engines[slot])
                 done = time.perf_counter()
+                lat = (done - job.arrival) * 1e3
+                kept = (res, lat) if self.reduce is None else self.reduce(job, res, lat)
                 with self.cv:
-                    self.results[job.job_id] = (res, (done - job.arrival) * 1e3)
+                    self.results[job.job_id] = kept
             except Exception as exc:  # reported per job, the server keeps going
                 with self.cv:
                     self.errors[job.job_id] = exc
@@ -217,6 +220,43 @@ class Server:
         if failed:
             raise RuntimeError(f"jobs failed: {failed}")
         return {i: self.results[i] for i in ids}
+
+    def stream(self, jobs, reduce, progress=None):
+        """Serve a stream of (arrival offset in seconds, Job) pairs, each
+        submitted at its arrival; jobs may be created lazily by the
+        iterator.  Only reduce(job, results, latency_ms) of each finished
+        job is kept, so a long stream holds no more than the jobs in the
+        queue.  progress(done, submitted) is called as jobs finish.
+        Returns {job_id: reduce(...)}."""
+        self.reduce = reduce
+        t0 = time.perf_counter()
+        ids = []
+        try:
+            for offset, job in jobs:
+                delay = t0 + offset - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                ids.append(job.job_id)
+                self.submit(job, arrival=t0 + offset)
+                if progress is not None and len(ids) % 100 == 0:
+                    with self.cv:
+                        progress(len(self.results), len(ids))
+            with self.cv:
+                last = -1
+                while True:
+                    done = sum(1 for i in ids if i in self.results or i in self.errors)
+                    if progress is not None and done != last:
+                        progress(done, len(ids))
+                        last = done
+                    if done == len(ids):
+                        break
+                    self.cv.wait(timeout=5.0)
+                failed = {i: self.errors[i] for i in ids if i in self.errors}
+        finally:
+            self.reduce = None
+        if failed:
+            raise RuntimeError(f"jobs failed: {failed}")
+        return {i: self.results.pop(i) for i in ids}
 
     def close(self):
         with self.cv:

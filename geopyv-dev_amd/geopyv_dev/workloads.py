@@ -325,21 +325,47 @@ def shear_band_pair(seed=2, shape=(3000, 4096), n_speckles=150000, n_subsets=150
                 order=2, p0=p0, truth=truth)
 
 
+_SEQ_PATTERNS = {}
+
+
+def _sequence_frame(spec):
+    """Frame k of sequence_frames (a worker of its process pool; the
+    pattern is rendered once per process)."""
+    seed, shape, n_frames, n_speckles, sigma, slip_max, k = spec
+    key = (seed, shape, n_speckles, sigma)
+    sp = _SEQ_PATTERNS.get(key)
+    if sp is None:
+        sp = _SEQ_PATTERNS[key] = Speckle(shape, n_speckles, sigma, seed)
+    h, w = shape
+    slip = slip_max * k / max(1, n_frames - 1)
+    field = ShearBand(((w - 1) / 2.0, (h - 1) / 2.0), slip=slip)
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, 1000 + k])))
+    return sp.target(field, 2.0, rng=rng), field
+
+
 def sequence_frames(seed=3, shape=(2048, 2048), n_frames=200, n_speckles=80000, sigma=2.0, slip_max=8.0,
-                    progress=None):
+                    progress=None, workers=1):
     """Frames of an incremental sequence: one speckle pattern (σ = 2 px)
     under a shear band whose slip grows linearly from 0 (frame 0) to
     slip_max (last frame); grey-level noise σ = 2 drawn per frame from
-    seed(frame).  Returns (frames, fields); frames are 8-bit arrays."""
-    h, w = shape
-    sp = Speckle(shape, n_speckles, sigma, seed)
+    seed(frame).  Returns (frames, fields); frames are 8-bit arrays.  With
+    workers > 1 the frames are rendered by that many spawned processes (the
+    same frames: each one's noise comes from its own seed)."""
+    specs = [(seed, tuple(shape), n_frames, n_speckles, sigma, slip_max, k) for k in range(n_frames)]
     frames, fields = [], []
-    for k in range(n_frames):
-        slip = slip_max * k / max(1, n_frames - 1)
-        field = ShearBand(((w - 1) / 2.0, (h - 1) / 2.0), slip=slip)
-        rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, 1000 + k])))
-        frames.append(sp.target(field, 2.0, rng=rng))
-        fields.append(field)
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(workers) as pool:
+            for k, (fr, fd) in enumerate(pool.imap(_sequence_frame, specs)):
+                frames.append(fr)
+                fields.append(fd)
+                if progress is not None:
+                    progress(k)
+        return frames, fields
+    for k, spec in enumerate(specs):
+        fr, fd = _sequence_frame(spec)
+        frames.append(fr)
+        fields.append(fd)
         if progress is not None:
             progress(k)
     return frames, fields
