@@ -804,6 +804,7 @@ __device__ __forceinline__ bool compose6_fast(const double* p, const double* dp,
 struct Plan {
     int x0, y0, ww, wh, cx_lo, cx_hi, cy_lo, cy_hi;
     int ls, lds, restage, fast;
+    int diag;  // predicted last iteration: the pass also sums the diagnostic's projections
     float q[12];  // q[2] = 1 + u_x, q[5] = 1 + v_y, second-order terms pre-halved
     float c0x, c0y;  // centre fraction + (u, v): the warped centre relative to the base
 };
@@ -844,6 +845,7 @@ __device__ __forceinline__ Plan make_plan(const double* p, const int32_t ext[4],
     P.lds = W.lds ? 1 : 0;
     P.restage = restage ? 1 : 0;
     P.fast = (box_ok && covers(W, c) && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax) ? 1 : 0;
+    P.diag = 0;
 #pragma unroll
     for (int k = 0; k < 12; ++k) P.q[k] = (k < M) ? (float)p[k] : 0.f;
     if (M >= 6) {
@@ -890,6 +892,111 @@ __device__ __forceinline__ Plan load_plan(const Plan* sp) {
     return P;
 }
 
+// One pass over the pixels at plan `pl` (its window W staged less f_m):
+// acc[0, 3 + M) gets s1, s2, s3, B_j, and with DIAG acc[3 + M, 3 + M + 2K)
+// gets the diagnostic's projection sums Σ c·mode·g' per gradient axis
+// (c = ∇f less its mean (mx, my), g' = g - f_m).  The two forms perform
+// the same FP32 operations on the iteration sums.
+template <int M, int P, int WPS, int S, bool DIAG>
+__device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Window& W, const float (&am)[P],
+                                           const float (&gxr)[P], const float (&gyr)[P], const OffXY* sXY, int n,
+                                           int tid, int bx, int by, float f_m32, float mx, float my,
+                                           const float* win, const BatchArgs& A) {
+    constexpr int NT = 64 * WPS;
+    constexpr int PU = P <= 4 ? P : 1;
+    constexpr int K = Modes<M>::K;
+    constexpr int NA = 3 + M;
+    constexpr int NV = DIAG ? NA + 2 * K : NA;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.f;
+    const auto diag = [&](float gp, float2 d, int k) {
+        if constexpr (DIAG) {
+            const float cx = gxr[k] - mx, cy = gyr[k] - my;
+            float mo[4];
+            modes32<M>(d.x, d.y, mo);
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const float t = gp * mo[q];
+                acc[NA + q] = fmaf(cx, t, acc[NA + q]);
+                acc[NA + K + q] = fmaf(cy, t, acc[NA + K + q]);
+            }
+        }
+    };
+    if (pl.fast) {
+        // positions less 1/2, rounded to nearest: the cell and the
+        // centred basis coordinate s in [-1/2, 1/2] in two operations
+        const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
+        const int LS = S > 0 ? S : W.ls;
+        const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
+        const float wwf = (float)LS;
+#pragma unroll PU
+        for (int k = 0; k < P; ++k) {
+            const int i = tid + NT * k;
+            if (i < n) {
+                const float2 d = off_xy(sXY[i]);
+                float ox, oy;
+                warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
+                const float rx = rintf(ox), ry = rintf(oy);
+                // offsets inside the window are small integers: exact in FP32
+                float cc[36];
+                gather_block<S>(win, cb + (int)fmaf(ry, wwf, rx), W.ls, cc);
+                float a[6], b[6];
+                basis_f32_s(oy - ry, a);
+                basis_f32_s(ox - rx, b);
+                const float gp = contract6r(cc, a, b);
+                accumulate<M>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
+                diag(gp, d, k);
+            }
+        }
+    } else {
+#pragma unroll PU
+        for (int k = 0; k < P; ++k) {
+            const int i = tid + NT * k;
+            if (i < n) {
+                const float2 d = off_xy(sXY[i]);
+                float ox, oy;
+                warp_pos<M>(pl.q, pl.c0x, pl.c0y, d.x, d.y, ox, oy);
+                int cx, cy;
+                float tx, ty;
+                cell_frac(bx, ox, A.w, cx, tx);
+                cell_frac(by, oy, A.h, cy, ty);
+                float d0, d1;
+                const float gp = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, f_m32, d0, d1);
+                accumulate<M>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
+                diag(gp, d, k);
+            }
+        }
+    }
+}
+
+// The warp-adequacy scores (subset.rs:748-820) from the projection sums
+// G_l = Σ c·mode·g' of the last iteration's pass (uniform), lane l < 2K
+// taking projection l = (axis l / K, mode l % K).  With z = c / sd, the
+// standardised column sc = (z·mode - rs) / spread and e the zero-normalised
+// residual, Σ sc·(e - ē) = (Σ c·mode·e / sd - rs Σe) / spread (Σ sc = 0),
+// and Σ c·mode·e = (Σ c·mode·a)/Δf - (G - (g_m - f_m')·Σ c·mode)/Δg, so
+// projection l = dc[l] - rg·(dc[2K + l]·G_l - mgp·dc[4K + l]) with the
+// reference-only terms dc from setup; mgp = g_m - f_m' (f_m' the FP32
+// shift of the window), rg = 1/Δg.  Returns η_u², η_v² (sums of squares
+// in mode order) in every lane.
+template <int K>
+__device__ __forceinline__ void diag_eta2(const double* G, const double* dc, double rg, double mgp, int lane,
+                                          double& eu, double& ev) {
+    double g = 0.0;
+#pragma unroll
+    for (int l = 0; l < 2 * K; ++l) g = (lane == l) ? G[l] : g;
+    const int l = lane < 2 * K ? lane : 0;
+    const double pr = dc[l] - rg * (dc[2 * K + l] * g - mgp * dc[4 * K + l]);
+    const double p2 = pr * pr;
+    eu = 0.0;
+    ev = 0.0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        eu += readlane_f64(p2, q);
+        ev += readlane_f64(p2, K + q);
+    }
+}
+
 // LIST: a list solve, reading the initial warps from A.p0_list and writing
 // the result records to A.res_list in dispatch order (a separate
 // instantiation, so whole-batch solves compile exactly as without it)
@@ -906,8 +1013,18 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     constexpr int NH1 = (M <= 6) ? NL : 0;             // Hessian entries summed in the first setup pass
     constexpr bool kHinv = true;                       // the update is one dot product per lane with H⁻¹
     constexpr int V1 = 2 + 2 * M + NH1;                // first setup pass
-    constexpr int V2 = 2;                              // second setup pass: Σa, Σa²
-    constexpr int VF = 4 + 6 * K;                      // final pass
+    constexpr int V2 = 4 + 6 * K;                      // second setup pass: f and the diagnostic's reference sums
+    constexpr int NV = NA + 2 * K;                     // a diagnostic pass: the iteration sums + projections
+    // The last iteration's pass also sums the diagnostic's projections when
+    // it is predicted: the previous update's norm below 6·max_norm (or the
+    // iteration cap next) foretells the last iteration for ~99 % of the
+    // subsets; a missed one runs that pass once more after the loop.  The
+    // second-order warp keeps its register budget and always runs it after.
+#ifndef GEODIC_PREDICT
+#define GEODIC_PREDICT 1
+#endif
+    constexpr bool kPredict = GEODIC_PREDICT && M <= 6;
+    constexpr double kDiagT2 = 36.0;
     GEODIC_DYN_LDS(float, lds);
     float* win = lds;
     OffXY* sXY = (OffXY*)(lds + A.win_cap);
@@ -916,13 +1033,15 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     __shared__ double sRd[M == 12 ? M : 1];
     __shared__ double sAS[2 * M];               // A_j, S_j
     __shared__ float sDg[2];                    // mean gradient mx, my
+    __shared__ double sDc[6 * K];               // the diagnostic's reference terms (diag_eta2)
+    __shared__ double sFin[1 + 2 * K];          // last iteration: ZNSSD, the projections' sums
     __shared__ double sP[12];                   // the current warp (its only copy)
     __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
     __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
     __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
-    __shared__ double sG[3];  // g_m, Δg, norm of the last update
+    __shared__ double sG[3];  // g_m, Δg, squared norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
-    __shared__ int sCtl[2];
+    __shared__ int sCtl[3];
     __shared__ Plan sPlan;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -1112,16 +1231,22 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             double p0[M];
 #pragma unroll
             for (int k = 0; k < M; ++k) p0[k] = readlane_f64(p0v, k);
-            const Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
+            Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
+            pl.diag = (kPredict && A.max_it == 1) ? 1 : 0;
             if (lane == 0) {
+                sCtl[2] = 0;
                 sPlan = pl;
                 for (int k = 0; k < M; ++k) sP[k] = p0[k];
             }
         }
 
-        // ---- second pass: a = f - f_m in place, and its sums (the
-        // diagnostic's reference statistics ride on the final pass) ----
+        // ---- second pass: a = f - f_m in place, Σa, Σa², and the
+        // diagnostic's reference sums: per gradient axis (∇f less its mean:
+        // c) Σc², Σ c·mode, Σ (c·mode)², Σ c·mode·a (subset.rs:698-727,
+        // 786-818) ----
         const float d32 = (float)dlt;
+        const float mx = uni((float)(w1[2 + M] / dn));  // Σ sdi_0 = Σ ∂x f
+        const float my = uni((float)(w1[3 + M] / dn));  // Σ sdi_1 = Σ ∂y f
         float v2[V2];
 #pragma unroll
         for (int j = 0; j < V2; ++j) v2[j] = 0.f;
@@ -1133,6 +1258,23 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 am[k] = a;
                 v2[0] += a;
                 v2[1] = fmaf(a, a, v2[1]);
+                const float cx = gxr[k] - mx, cy = gyr[k] - my;
+                v2[2] = fmaf(cx, cx, v2[2]);
+                v2[3] = fmaf(cy, cy, v2[3]);
+                const float2 d = off_xy(sXY[i]);
+                float mo[4];
+                modes32<M>(d.x, d.y, mo);
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    // explicit FMAs throughout: every instantiation rounds alike
+                    const float xm = cx * mo[q], ym = cy * mo[q];
+                    v2[4 + q] += xm;
+                    v2[4 + K + q] += ym;
+                    v2[4 + 2 * K + q] = fmaf(xm, xm, v2[4 + 2 * K + q]);
+                    v2[4 + 3 * K + q] = fmaf(ym, ym, v2[4 + 3 * K + q]);
+                    v2[4 + 4 * K + q] = fmaf(xm, a, v2[4 + 4 * K + q]);
+                    v2[4 + 5 * K + q] = fmaf(ym, a, v2[4 + 5 * K + q]);
+                }
             }
         }
         double w2[V2];
@@ -1140,11 +1282,37 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         sa = w2[0];
         const double sq = w2[1];
         delta_f = uni(sqrt(sq));
+        if (wave == 0) {
+            // the diagnostic's reference terms, projection l on lane l: with
+            // sd the gradient's spread, rs and spread the mean and spread of
+            // z·mode (z = c / sd), α = 1/(sd·spread·n), β = rs/(spread·n):
+            // dc = (α Σ c·mode·a - β Σa)/Δf, α, α Σ c·mode
+            if (lane < 2 * K) {
+                const double rn = 1.0 / dn;
+                const double isd = 1.0 / sqrt(fmax((lane < K ? w2[2] : w2[3]) * rn, 1e-18));
+                double scm = 0.0, scm2 = 0.0, scma = 0.0;
+#pragma unroll
+                for (int j = 0; j < 2 * K; ++j) {
+                    if (j == lane) {
+                        scm = w2[4 + j];
+                        scm2 = w2[4 + 2 * K + j];
+                        scma = w2[4 + 4 * K + j];
+                    }
+                }
+                const double rs = scm * isd * rn;
+                const double var = scm2 * isd * isd * rn - rs * rs;
+                const double isp = 1.0 / sqrt(fmax(var, 1e-18));
+                const double al = isd * isp * rn, be = rs * isp * rn;
+                sDc[lane] = (al * scma - be * sa) / delta_f;
+                sDc[2 * K + lane] = al;
+                sDc[4 * K + lane] = al * scm;
+            }
+        }
         if (tid == 0) {
             R->delta_f = delta_f;
             R->sigma = sqrt(sq / dn);
-            sDg[0] = (float)(w1[2 + M] / dn);  // Σ sdi_0 = Σ ∂x f
-            sDg[1] = (float)(w1[3 + M] / dn);  // Σ sdi_1 = Σ ∂y f
+            sDg[0] = mx;
+            sDg[1] = my;
         }
     }
     if (delta_f == 0.0) {
@@ -1180,57 +1348,32 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             stage_window<NT>(win, A.gC, A.pitch, W, tid, f_m32);
             __syncthreads();
         }
-        float acc[NA];
-#pragma unroll
-        for (int j = 0; j < NA; ++j) acc[j] = 0.f;
-        if (pl.fast) {
-            // positions less 1/2, rounded to nearest: the cell and the
-            // centred basis coordinate s in [-1/2, 1/2] in two operations
-            const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
-            const int LS = S > 0 ? S : W.ls;
-            const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
-            const float wwf = (float)LS;
-#pragma unroll PU
-            for (int k = 0; k < P; ++k) {
-                const int i = tid + NT * k;
-                if (i < n) {
-                    const float2 d = off_xy(sXY[i]);
-                    float ox, oy;
-                    warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
-                    const float rx = rintf(ox), ry = rintf(oy);
-                    // offsets inside the window are small integers: exact in FP32
-                    float cc[36];
-                    gather_block<S>(win, cb + (int)fmaf(ry, wwf, rx), W.ls, cc);
-                    float a[6], b[6];
-                    basis_f32_s(oy - ry, a);
-                    basis_f32_s(ox - rx, b);
-                    accumulate<M>(acc, contract6r(cc, a, b), am[k], gxr[k], gyr[k], d.x, d.y);
-                }
-            }
-        } else {
-#pragma unroll PU
-            for (int k = 0; k < P; ++k) {
-                const int i = tid + NT * k;
-                if (i < n) {
-                    const float2 d = off_xy(sXY[i]);
-                    float ox, oy;
-                    warp_pos<M>(pl.q, pl.c0x, pl.c0y, d.x, d.y, ox, oy);
-                    int cx, cy;
-                    float tx, ty;
-                    cell_frac(bx, ox, A.w, cx, tx);
-                    cell_frac(by, oy, A.h, cy, ty);
-                    float d0, d1;
-                    const float gp = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, f_m32, d0, d1);
-                    accumulate<M>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
-                }
-            }
-        }
-        double sum[NA];
-#pragma unroll
-        for (int j = 0; j < NA; ++j) sum[j] = (double)acc[j];
         // the lead wavefront alone runs the update; the others wait at the
         // barrier below, leaving their SIMD slots to other workgroups
-        if (block_sum_lead<WPS, NA>(sum, sRed, lane, wave)) {
+        const bool dg_it = kPredict && pl.diag;
+        double sum[NV];
+        bool lead;
+        if (dg_it) {
+            float acc[NV];
+            if constexpr (kPredict)
+                pixel_pass<M, P, WPS, S, true>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, sDg[0], sDg[1],
+                                               win, A);
+#pragma unroll
+            for (int j = 0; j < NV; ++j) sum[j] = (double)acc[j];
+            lead = block_sum_lead<WPS, NV>(sum, sRed, lane, wave);
+            // the projections' sums wait in LDS in case this pass is the last
+            if (lead && lane == 0) {
+#pragma unroll
+                for (int l = 0; l < 2 * K; ++l) sFin[1 + l] = sum[NA + l];
+            }
+        } else {
+            float acc[NA];
+            pixel_pass<M, P, WPS, S, false>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, 0.f, 0.f, win, A);
+#pragma unroll
+            for (int j = 0; j < NA; ++j) sum[j] = (double)acc[j];
+            lead = block_sum_lead<WPS, NA>(sum, sRed, lane, wave);
+        }
+        if (lead) {
             const double f_m = sK[0], delta_f = sK[1], sa = sK[2], rn = sK[3], size = sK[4];
             const double max_norm2 = A.max_norm * A.max_norm;
             // g' above is g - (float)f_m; fold the rounding of the shift back in
@@ -1289,9 +1432,17 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 nrm2 = convergence_norm2_dev<M>(dp, size);
             }
             const bool conv = ok && nrm2 < max_norm2;
-            if (ok && A.hist && it <= A.hist_cap && lane == 0) {
+            const bool last = ok && (conv || it == A.max_it);
+            const bool keep_h = ok && A.hist && it <= A.hist_cap;
+            // ZNSSD at this iteration's warp (subset.rs:418-446 in sums)
+            double znssd = 0.0;
+            if (last || keep_h) znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
+            if (last && lane == 0) {
+                sFin[0] = znssd;
+                if (dg_it) sCtl[2] = 1;  // its projections' sums are in sFin
+            }
+            if (keep_h && lane == 0) {
                 const double nrm = sqrt(nrm2);
-                const double znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
                 double* hh = A.hist + ((int64_t)s * A.hist_cap + (it - 1)) * 4;
                 hh[0] = (double)it;
                 hh[1] = nrm;
@@ -1304,6 +1455,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
 #pragma unroll
                 for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
                 nx = make_plan<M, S>(pn, ext, xc, yc, fxc, fyc, W, A);
+                nx.diag = (kPredict && (nrm2 < kDiagT2 * max_norm2 || it + 1 == A.max_it)) ? 1 : 0;
             }
             if (lane == 0) {
 #pragma unroll
@@ -1344,96 +1496,32 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         last_norm = uni(sqrt(sG[2]));
     }
 
-    // ---- final ZNSSD (subset.rs:1190-1193) and the diagnostic's projections
-    // on g at the last iteration's warp (its plan and staged window are still
-    // current); with no iteration g is all zero and the reference's NaN
-    // propagates alike ----
-    const float rf = uni((float)(1.0 / sK[1]));
-    const float rg = it_done ? uni((float)(1.0 / delta_g)) : __int_as_float(0x7fc00000);
-    const float mg32 = it_done ? uni((float)(g_m - sK[0])) : -f_m32;
-    // Σe², Σe, then per gradient axis (∇f less its mean: c) Σc², Σ c·mode,
-    // Σ (c·mode)², Σ c·mode·e: the diagnostic's gradient spreads, mode means
-    // and mode spreads (subset.rs:698-727, 786-818) and its projections
-    float fin[VF];
+    // ---- the diagnostic (subset.rs:1196-1210) on g at the last iteration's
+    // warp: its projections were summed in that pass when it was predicted,
+    // else that pass runs once more here (its plan and staged window are
+    // still current) with the same operations; with no iteration g is all
+    // zero and the reference's NaN propagates alike ----
+    double eu = 0.0, ev = 0.0;
+    if (it_done) {
+        double sum[NV];
+        bool lead = wave == 0;
+        if (!sCtl[2]) {
+            const Window W = plan_window(pl);
+            float acc[NV];
+            pixel_pass<M, P, WPS, S, true>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, sDg[0], sDg[1], win,
+                                           A);
 #pragma unroll
-    for (int j = 0; j < VF; ++j) fin[j] = 0.f;
-    {
-        const float mx = sDg[0], my = sDg[1];
-        const Window W = plan_window(pl);
-        const bool fast = it_done && pl.fast;
-        const float c0xh = uni(pl.c0x - 0.5f), c0yh = uni(pl.c0y - 0.5f);
-        const int LS = S > 0 ? S : W.ls;
-        const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
-#pragma unroll PU
-        for (int k = 0; k < P; ++k) {
-            const int i = tid + NT * k;
-            if (i < n) {
-                const float2 d = off_xy(sXY[i]);
-                float gp = 0.f;  // g - f_m
-                if (fast) {
-                    float ox, oy;
-                    warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
-                    const float rx = rintf(ox), ry = rintf(oy);
-                    float cc[36];
-                    gather_block<S>(win, cb + (int)fmaf(ry, (float)LS, rx), W.ls, cc);
-                    float a[6], b[6];
-                    basis_f32_s(oy - ry, a);
-                    basis_f32_s(ox - rx, b);
-                    gp = contract6r(cc, a, b);
-                } else if (it_done) {
-                    float ox, oy;
-                    warp_pos<M>(pl.q, pl.c0x, pl.c0y, d.x, d.y, ox, oy);
-                    int cx, cy;
-                    float tx, ty;
-                    cell_frac(bx, ox, A.w, cx, tx);
-                    cell_frac(by, oy, A.h, cy, ty);
-                    float d0, d1;
-                    gp = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, f_m32, d0, d1);
-                }
-                const float e = fmaf(am[k], rf, -(gp - mg32) * rg);
-                fin[0] = fmaf(e, e, fin[0]);
-                fin[1] += e;
-                const float cx = gxr[k] - mx, cy = gyr[k] - my;
-                fin[2] = fmaf(cx, cx, fin[2]);
-                fin[3] = fmaf(cy, cy, fin[3]);
-                float mo[4];
-                modes32<M>(d.x, d.y, mo);
+            for (int j = 0; j < NV; ++j) sum[j] = (double)acc[j];
+            lead = block_sum_lead<WPS, NV>(sum, sRed, lane, wave);
+        } else if (lead) {
 #pragma unroll
-                for (int qq = 0; qq < K; ++qq) {
-                    // explicit FMAs throughout: every instantiation rounds alike
-                    const float xm = cx * mo[qq], ym = cy * mo[qq];
-                    fin[4 + qq] = fmaf(cx, mo[qq], fin[4 + qq]);
-                    fin[4 + K + qq] = fmaf(cy, mo[qq], fin[4 + K + qq]);
-                    fin[4 + 2 * K + qq] = fmaf(xm, xm, fin[4 + 2 * K + qq]);
-                    fin[4 + 3 * K + qq] = fmaf(ym, ym, fin[4 + 3 * K + qq]);
-                    fin[4 + 4 * K + qq] = fmaf(xm, e, fin[4 + 4 * K + qq]);
-                    fin[4 + 5 * K + qq] = fmaf(ym, e, fin[4 + 5 * K + qq]);
-                }
-            }
+            for (int l = 0; l < 2 * K; ++l) sum[NA + l] = sFin[1 + l];
         }
+        if (lead) diag_eta2<K>(sum + NA, sDc, 1.0 / delta_g, g_m - (double)f_m32, lane, eu, ev);
     }
-    double fs[VF];
-    block_sum32<WPS, VF>(fin, fs, sRed, lane, wave);
     if (tid == 0) {
-        const double znssd = it_done ? fs[0] : 4.0;
+        const double znssd = it_done ? sFin[0] : 4.0;
         const double zncc = it_done ? 1.0 - znssd / 2.0 : 0.0;
-        // with z = c / sd (sd the gradient's spread), the standardised mode
-        // sc = (z·mode - rs) / spread (rs, spread: mean and spread of
-        // z·mode); Σ sc·(e - ē) = (Σ c·mode·e / sd - rs Σe) / spread, as
-        // Σ sc = 0 the centring of e drops out
-        double eu = 0.0, ev = 0.0;
-#pragma unroll
-        for (int ax = 0; ax < 2; ++ax) {
-            const double sd = sqrt(fmax(fs[2 + ax] / dn, 1e-18));
-            double ss = 0.0;
-            for (int qq = 0; qq < K; ++qq) {
-                const double rs = fs[4 + ax * K + qq] / (sd * dn);
-                const double var = fs[4 + (2 + ax) * K + qq] / (sd * sd * dn) - rs * rs;
-                const double pr = (fs[4 + (4 + ax) * K + qq] / sd - rs * fs[1]) / (sqrt(fmax(var, 1e-18)) * dn);
-                ss += pr * pr;
-            }
-            (ax == 0 ? eu : ev) = ss;
-        }
         for (int k = 0; k < 12; ++k) {
             const double pk = k < M ? sP[k] : 0.0;
             R->p[k] = pk;

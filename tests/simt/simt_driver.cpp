@@ -209,5 +209,11 @@ int main(int argc, char** argv) {
     }
     printf("order %d radius %d subsets %d: max|du| %.3e max|dstrain| %.3e mismatched %d\n", order, radius, n,
            dumax, dsmax, bad);
+    if (argc > 8) {
+        // the raw result records, for bit-for-bit comparisons between builds
+        FILE* fo = fopen(argv[8], "wb");
+        if (!fo || fwrite(res.data(), sizeof(dic_subset_result), res.size(), fo) != res.size()) return 2;
+        fclose(fo);
+    }
     return bad ? 1 : 0;
 }
