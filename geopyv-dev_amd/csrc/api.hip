@@ -322,6 +322,7 @@ static void batch_free(Batch* b) {
     dev_free(e, b->d_hist, sizeof(double) * 4 * n * b->hist_cap);
     dev_free(e, b->d_scratch, sizeof(double) * b->scratch_doubles);
     dev_free(e, b->d_order_perm, sizeof(int32_t) * n);
+    dev_free(e, b->d_by_order, sizeof(int32_t) * (b->mixed_orders ? n : 0));
     if (b->h_list_io) (void)hipHostFree(b->h_list_io);
 }
 
@@ -383,11 +384,13 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     CHECK_ARG(tpl_start[0] == 0, "invalid input: tpl_start[0] must be 0");
     for (int t = 0; t < n_templates; ++t)
         CHECK_ARG(tpl_start[t + 1] >= tpl_start[t], "invalid input: tpl_start must be non-decreasing");
-    const int order0 = orders[0];
+    int32_t max_order = 0;
+    bool mixed = false;
     for (int64_t s = 0; s < n_subsets; ++s) {
         CHECK_ARG(orders[s] >= 0 && orders[s] <= 2, "invalid input: subset_order must be 0, 1, or 2");
-        CHECK_ARG(orders[s] == order0, "invalid input: all subsets of one batch share the warp order");
         CHECK_ARG(tpl_index[s] >= 0 && tpl_index[s] < n_templates, "invalid input: template index out of range");
+        max_order = std::max(max_order, orders[s]);
+        mixed = mixed || orders[s] != orders[0];
     }
     Engine* e = &d->e;
     DIC_HIP_TRY(hipSetDevice(e->device));
@@ -397,7 +400,9 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     b->eng = e;
     b->n = n_subsets;
     b->n_tpl = n_templates;
-    b->max_order = order0;
+    b->max_order = max_order;
+    b->mixed_orders = mixed;
+    b->h_orders.assign(orders, orders + n_subsets);
     b->h_tpl_start.assign(tpl_start, tpl_start + n_templates + 1);
     b->h_tpl_index.assign(tpl_index, tpl_index + n_subsets);
     const int64_t n_off = tpl_start[n_templates];
@@ -436,15 +441,28 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     }
     b->h_coords.assign(coords, coords + 2 * n_subsets);
     b->h_ext = ext;
-    // provisional dispatch order (largest subsets first); refined for the
-    // image shape at the first solve
+    // provisional dispatch order (highest warp order, then largest subsets
+    // first); refined for the image shape at the first solve
     std::vector<int32_t> perm(n_subsets);
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t c) {
+        if (orders[a] != orders[c]) return orders[a] > orders[c];
         const int64_t na = tpl_start[tpl_index[a] + 1] - tpl_start[tpl_index[a]];
         const int64_t nc = tpl_start[tpl_index[c] + 1] - tpl_start[tpl_index[c]];
         return na > nc;
     });
+    // mixed orders: the subsets grouped by order for the plain kernel's
+    // one launch per order
+    std::vector<int32_t> by_order;
+    if (mixed) {
+        by_order.reserve(n_subsets);
+        for (int o = 0; o < 3; ++o) {
+            b->by_order_off[o] = (int64_t)by_order.size();
+            for (int64_t s = 0; s < n_subsets; ++s)
+                if (orders[s] == o) by_order.push_back((int32_t)s);
+        }
+        b->by_order_off[3] = n_subsets;
+    }
     int st = DIC_OK;
     auto A = [&](void** p, size_t bytes) {
         if (!st) st = dev_alloc(e, p, bytes);
@@ -461,6 +479,7 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     A((void**)&b->d_p, sizeof(double) * 12 * n_subsets);
     A((void**)&b->d_res, sizeof(dic_subset_result) * n_subsets);
     A((void**)&b->d_order_perm, sizeof(int32_t) * n_subsets);
+    if (mixed) A((void**)&b->d_by_order, sizeof(int32_t) * n_subsets);
     A(&b->d_meta, kMetaBytes * n_subsets);
     auto H2D = [&](void* dst, const void* src, size_t bytes) {
         if (!st && bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess) {
@@ -479,6 +498,7 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     H2D(b->d_tpl_xy_eng, packed_eng.data(), sizeof(int32_t) * n_off);
     H2D(b->d_tpl_ext, ext.data(), sizeof(int32_t) * 4 * n_templates);
     H2D(b->d_order_perm, perm.data(), sizeof(int32_t) * n_subsets);
+    if (mixed) H2D(b->d_by_order, by_order.data(), sizeof(int32_t) * n_subsets);
     H2D(b->d_scr_off, b->h_scr_off.data(), sizeof(int64_t) * n_subsets);
     // the batched engine's start record per subset (centre, template
     // start, size and extents)
@@ -549,8 +569,9 @@ int dic_batch_set_schedule(dic_batch* db, int win_slack, int waves_per_subset) {
 
 namespace {
 
-// Dispatch order for an image of w x h.  Largest subsets first.  Within
-// a size, the subsets (in their given, usually raster, order) are cut into
+// Dispatch order for an image of w x h.  Highest warp order first (each
+// order's classes are separate launches), then largest subsets first.
+// Within an order and a size, the subsets (in their given, usually raster, order) are cut into
 // eight contiguous bands and dealt out so that dispatch slot b takes band
 // b mod 8: the hardware sends workgroup b to XCD b mod 8, so each XCD works
 // through one band and neighbouring subsets share its L2 (an eighth of the
@@ -574,12 +595,16 @@ int refresh_order(Batch* b, int64_t w, int64_t h) {
     }
     std::vector<int32_t> by_size(n);
     std::iota(by_size.begin(), by_size.end(), 0);
-    std::stable_sort(by_size.begin(), by_size.end(), [&](int32_t a, int32_t c) { return npx[a] > npx[c]; });
+    const std::vector<int32_t>& ord = b->h_orders;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int32_t a, int32_t c) {
+        if (ord[a] != ord[c]) return ord[a] > ord[c];
+        return npx[a] > npx[c];
+    });
     b->h_perm.clear();
     b->h_perm.reserve(n);
     for (int64_t g0 = 0; g0 < n;) {
         int64_t g1 = g0;
-        while (g1 < n && npx[by_size[g1]] == npx[by_size[g0]]) ++g1;
+        while (g1 < n && npx[by_size[g1]] == npx[by_size[g0]] && ord[by_size[g1]] == ord[by_size[g0]]) ++g1;
         const int64_t len = g1 - g0;
         std::vector<std::vector<int32_t>> band(8);
         for (int x = 0; x < 8; ++x) {
@@ -679,7 +704,7 @@ int dic_batch_solve_noalloc(dic_batch* db, const dic_image* f, const dic_image* 
         return DIC_ERR_DEVICE;
     }
     if (!e->capturing) DIC_HIP_TRY(hipEventRecord(e->ev_sol0, e->stream));
-    int st = launch_icgn_plain(b, &f->im, &g->im, prm, nullptr, 0, nullptr, nullptr, ptr,
+    int st = launch_icgn_plain(b, &f->im, &g->im, prm, nullptr, nullptr, 0, nullptr, nullptr, ptr,
                                kNoallocScratchBytes / sizeof(double));
     if (st) return st;
     if (!e->capturing) {
@@ -721,14 +746,15 @@ int dic_batch_get_schedule(dic_batch* db, int32_t* wps, int32_t* slack) {
     CHECK_ARG(db && wps && slack, "dic_batch_get_schedule: NULL argument");
     Batch* b = &db->b;
     std::lock_guard<std::mutex> lk(b->eng->mu);
+    const int o = b->max_order;
     for (int k = 0; k < 4; ++k) {
         const int base = 1 << k;
         if (b->waves_per_subset > 0) {
             wps[k] = b->waves_per_subset;
             slack[k] = b->win_slack;
-        } else if (b->tuned_wps[base] > 0) {
-            wps[k] = b->tuned_wps[base];
-            slack[k] = b->tuned_slack[base];
+        } else if (b->tuned_wps[o][base] > 0) {
+            wps[k] = b->tuned_wps[o][base];
+            slack[k] = b->tuned_slack[o][base];
         } else {
             wps[k] = base;
             slack[k] = b->win_slack;
@@ -816,6 +842,21 @@ int dic_batch_residual_map(dic_batch* db, const dic_image* f, const dic_image* g
         return DIC_ERR_NOT_SOLVED;
     }
     if (count == 0) return DIC_OK;
+    // launched grouped by warp order (one launch per order); each listed
+    // subset's row stays where the list puts it
+    std::vector<int32_t> pos(count);
+    std::iota(pos.begin(), pos.end(), 0);
+    if (b->mixed_orders)
+        std::stable_sort(pos.begin(), pos.end(),
+                         [&](int32_t a, int32_t c) { return b->h_orders[idx[a]] > b->h_orders[idx[c]]; });
+    std::vector<int32_t> idx_s(count);
+    std::vector<int64_t> off_s(count);
+    std::vector<double> p_s(p ? 12 * (size_t)count : 0);
+    for (int64_t q = 0; q < count; ++q) {
+        idx_s[q] = idx[pos[q]];
+        off_s[q] = off[pos[q]];
+        if (p) std::memcpy(&p_s[12 * q], p + 12 * (int64_t)pos[q], sizeof(double) * 12);
+    }
     // one device block: list, offsets, warps (when given), residuals
     const size_t list_b = ((size_t)count * 4 + 7) & ~(size_t)7;
     const size_t off_b = sizeof(int64_t) * (size_t)count;
@@ -830,13 +871,19 @@ int dic_batch_residual_map(dic_batch* db, const dic_image* f, const dic_image* g
         dev_free(e, d, list_b + off_b + p_b + out_b);
         return DIC_ERR_DEVICE;
     };
-    if (hipMemcpyAsync(d, idx, sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
-        hipMemcpyAsync(d + list_b, off.data(), off_b, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
-        (p && hipMemcpyAsync(d + list_b + off_b, p, p_b, hipMemcpyHostToDevice, e->stream) != hipSuccess))
+    if (hipMemcpyAsync(d, idx_s.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+        hipMemcpyAsync(d + list_b, off_s.data(), off_b, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+        (p && hipMemcpyAsync(d + list_b + off_b, p_s.data(), p_b, hipMemcpyHostToDevice, e->stream) != hipSuccess))
         return fail("upload failed");
-    st = launch_residual(b, &f->im, &g->im, reinterpret_cast<const int32_t*>(d), count,
-                         p ? reinterpret_cast<const double*>(d + list_b + off_b) : b->d_p, p == nullptr,
-                         reinterpret_cast<const int64_t*>(d + list_b), reinterpret_cast<double*>(d + list_b + off_b + p_b));
+    const int32_t* d_list = reinterpret_cast<const int32_t*>(d);
+    const int64_t* d_off = reinterpret_cast<const int64_t*>(d + list_b);
+    const double* d_pl = reinterpret_cast<const double*>(d + list_b + off_b);
+    double* d_out = reinterpret_cast<double*>(d + list_b + off_b + p_b);
+    for (const OrderRun& r : order_runs(b, idx_s.data(), count)) {
+        st = launch_residual(b, &f->im, &g->im, r.order, d_list + r.start, r.count, p ? d_pl + 12 * r.start : b->d_p,
+                             p == nullptr, d_off + r.start, d_out);
+        if (st) break;
+    }
     if (st) {
         (void)hipStreamSynchronize(e->stream);
         dev_free(e, d, list_b + off_b + p_b + out_b);
@@ -1028,31 +1075,40 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
     dic_subset_result* d_res = reinterpret_cast<dic_subset_result*>(b->d_list_io + res_off);
     // the kernels read each listed subset's warp from the staging and write
     // its result record there, in their dispatch order: the list's own order
-    // for the plain kernel, by template size (largest first) for the
-    // batched engine, whose size classes need that
+    // within each warp order for the plain kernel (one launch per order),
+    // by warp order and then template size (largest first) for the batched
+    // engine, whose size classes need that
     int st = DIC_OK;
+    std::vector<int32_t>& pos = b->list_pos;
+    pos.resize(count);
+    for (int64_t k = 0; k < count; ++k) pos[k] = (int32_t)k;
+    const std::vector<int32_t>& ord = b->h_orders;
     if (prm->precision == DIC_PREC_PLAIN_F64) {
-        std::memcpy(h_idx, idx, sizeof(int32_t) * count);
-        std::memcpy(h_p0, p0, sizeof(double) * 12 * count);
+        if (b->mixed_orders)
+            std::stable_sort(pos.begin(), pos.end(), [&](int32_t a, int32_t c) { return ord[idx[a]] > ord[idx[c]]; });
+        for (int64_t q = 0; q < count; ++q) {
+            h_idx[q] = idx[pos[q]];
+            std::memcpy(h_p0 + 12 * q, p0 + 12 * (int64_t)pos[q], sizeof(double) * 12);
+        }
         // the listed subsets' per-pixel arrays overwrite those of the
         // stored whole-batch results
         b->arrays_valid = false;
-        st = launch_icgn_plain(b, f, g, prm, d_idx, count, d_src, d_res);
+        st = launch_icgn_plain(b, f, g, prm, d_idx, h_idx, count, d_src, d_res);
         if (st) {
             (void)hipStreamSynchronize(e->stream);  // nothing may still write the staging
             return st;
         }
         DIC_HIP_TRY(hipStreamSynchronize(e->stream));
-        std::memcpy(out, h_res, sizeof(dic_subset_result) * count);
+        for (int64_t q = 0; q < count; ++q) out[pos[q]] = h_res[q];
     } else {
-        std::vector<int32_t>& pos = b->list_pos;
-        pos.resize(count);
-        for (int64_t k = 0; k < count; ++k) pos[k] = (int32_t)k;
         auto npx = [&](int32_t k) {
             const int t = b->h_tpl_index[idx[k]];
             return b->h_tpl_start[t + 1] - b->h_tpl_start[t];
         };
-        std::stable_sort(pos.begin(), pos.end(), [&](int32_t a, int32_t c) { return npx(a) > npx(c); });
+        std::stable_sort(pos.begin(), pos.end(), [&](int32_t a, int32_t c) {
+            if (ord[idx[a]] != ord[idx[c]]) return ord[idx[a]] > ord[idx[c]];
+            return npx(a) > npx(c);
+        });
         for (int64_t q = 0; q < count; ++q) {
             h_ord[q] = idx[pos[q]];
             std::memcpy(h_p0 + 12 * q, p0 + 12 * (int64_t)pos[q], sizeof(double) * 12);

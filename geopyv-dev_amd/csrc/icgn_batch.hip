@@ -63,9 +63,10 @@ int min_wps(int64_t n) { return n <= 64 * 12 ? 1 : (n <= 128 * 12 ? 2 : (n <= 25
 
 }  // namespace
 
-// Size classes of a batch in dispatch order (the order sorts by size, so
-// each class is a contiguous run): subsets that need the same number of
-// wavefronts launch together; an explicit schedule makes one class.
+// Size classes of a batch in dispatch order (the order sorts by warp order,
+// then size, so each class is a contiguous run): subsets of one warp order
+// that need the same number of wavefronts launch together; an explicit
+// schedule makes one class per warp order.
 std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm, int64_t n) {
     if (!h_perm) {
         h_perm = b->h_perm.data();
@@ -82,18 +83,21 @@ std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm, int64
     };
     for (int64_t i = 0; i < n;) {
         const int w = b->waves_per_subset > 0 ? b->waves_per_subset : min_wps(npx(h_perm[i]));
+        const int o = b->h_orders[h_perm[i]];
         SizeClass c;
         c.start = i;
+        c.order = o;
         c.base_wps = w;
         c.wps = w;
         c.slack = b->win_slack;
-        if (b->waves_per_subset == 0 && b->tuned_wps[w] > 0) {
-            c.wps = b->tuned_wps[w];
-            c.slack = b->tuned_slack[w];
+        if (b->waves_per_subset == 0 && b->tuned_wps[o][w] > 0) {
+            c.wps = b->tuned_wps[o][w];
+            c.slack = b->tuned_slack[o][w];
         }
         c.max_n = 0;
         c.max_side = 0;
-        while (i < n && (b->waves_per_subset > 0 || min_wps(npx(h_perm[i])) == w)) {
+        while (i < n && b->h_orders[h_perm[i]] == o &&
+               (b->waves_per_subset > 0 || min_wps(npx(h_perm[i])) == w)) {
             c.max_n = std::max<int64_t>(c.max_n, npx(h_perm[i]));
             c.max_side = std::max(c.max_side, side(h_perm[i]));
             ++i;
@@ -150,8 +154,8 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
     Ac.win_cap = (s72 ? 72 : row) * side;
     const size_t lds = sizeof(float) * (size_t)Ac.win_cap + 4 * (size_t)c.max_n;
     const bool few = A.res_list && c.max_n <= 64 * 4 * c.wps;
-    if (b->max_order == 0) return launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st, few);
-    if (b->max_order == 1) return launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st, few);
+    if (c.order == 0) return launch_m<2>(Ac, c.wps, (int)c.count, lds, s72, st, few);
+    if (c.order == 1) return launch_m<6>(Ac, c.wps, (int)c.count, lds, s72, st, few);
     return launch_m<12>(Ac, c.wps, (int)c.count, lds, s72, st, few);
 }
 
@@ -200,7 +204,8 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
     Engine* e = b->eng;
     const BatchArgs A = common_args(b, f, g, prm);
-    for (int w = 0; w < 9; ++w) b->tuned_wps[w] = 0;
+    for (int o = 0; o < 3; ++o)
+        for (int w = 0; w < 9; ++w) b->tuned_wps[o][w] = 0;
     const int saved = b->waves_per_subset;
     b->waves_per_subset = 0;
     const std::vector<SizeClass> cls = size_classes(b);
@@ -241,8 +246,8 @@ int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solv
             }
         }
         if (rc) break;
-        b->tuned_wps[c0.base_wps] = bw;
-        b->tuned_slack[c0.base_wps] = bs;
+        b->tuned_wps[c0.order][c0.base_wps] = bw;
+        b->tuned_slack[c0.order][c0.base_wps] = bs;
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

@@ -543,9 +543,21 @@ __global__ void __launch_bounds__(64) icgn_plain_kernel(PlainArgs A) {
 
 }  // namespace
 
+std::vector<OrderRun> order_runs(const Batch* b, const int32_t* h, int64_t count) {
+    std::vector<OrderRun> runs;
+    for (int64_t k = 0; k < count;) {
+        const int o = b->h_orders[h[k]];
+        int64_t j = k;
+        while (j < count && b->h_orders[h[j]] == o) ++j;
+        runs.push_back({k, j - k, o});
+        k = j;
+    }
+    return runs;
+}
+
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* list,
-                      int64_t count, const double* p0_list, dic_subset_result* res_list, double* ext_scratch,
-                      size_t ext_doubles) {
+                      const int32_t* h_list, int64_t count, const double* p0_list, dic_subset_result* res_list,
+                      double* ext_scratch, size_t ext_doubles) {
     Engine* e = b->eng;
     const int arrays = plain_arrays(prm->method);
     const size_t need = (size_t)b->total_px * arrays;
@@ -594,21 +606,46 @@ int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.max_it = prm->max_iterations;
     A.tol = prm->tolerance;
     A.max_norm = prm->max_norm;
-    // all subsets of one batch share the warp order (checked at creation)
-    const int order = b->max_order;
     if (A.n_sub == 0) return DIC_OK;
-    const dim3 grid((unsigned)A.n_sub), block(64);
-    const bool lst = A.res_list != nullptr;
-    if (order == 0)
-        lst ? icgn_plain_kernel<2, true><<<grid, block, 0, e->stream>>>(A)
-            : icgn_plain_kernel<2, false><<<grid, block, 0, e->stream>>>(A);
-    else if (order == 1)
-        lst ? icgn_plain_kernel<6, true><<<grid, block, 0, e->stream>>>(A)
-            : icgn_plain_kernel<6, false><<<grid, block, 0, e->stream>>>(A);
-    else
-        lst ? icgn_plain_kernel<12, true><<<grid, block, 0, e->stream>>>(A)
-            : icgn_plain_kernel<12, false><<<grid, block, 0, e->stream>>>(A);
-    DIC_HIP_TRY(hipGetLastError());
+    // one launch per warp order: the whole batch in one launch when it has
+    // one order, else its subsets grouped by order; a list by its runs
+    auto launch = [&](int order, const PlainArgs& Ao) {
+        const dim3 grid((unsigned)Ao.n_sub), block(64);
+        const bool lst = Ao.res_list != nullptr;
+        if (order == 0)
+            lst ? icgn_plain_kernel<2, true><<<grid, block, 0, e->stream>>>(Ao)
+                : icgn_plain_kernel<2, false><<<grid, block, 0, e->stream>>>(Ao);
+        else if (order == 1)
+            lst ? icgn_plain_kernel<6, true><<<grid, block, 0, e->stream>>>(Ao)
+                : icgn_plain_kernel<6, false><<<grid, block, 0, e->stream>>>(Ao);
+        else
+            lst ? icgn_plain_kernel<12, true><<<grid, block, 0, e->stream>>>(Ao)
+                : icgn_plain_kernel<12, false><<<grid, block, 0, e->stream>>>(Ao);
+        return hipGetLastError();
+    };
+    if (!b->mixed_orders) {
+        DIC_HIP_TRY(launch(b->max_order, A));
+    } else if (!list) {
+        for (int o = 0; o < 3; ++o) {
+            PlainArgs Ao = A;
+            Ao.list = b->d_by_order + b->by_order_off[o];
+            Ao.n_sub = b->by_order_off[o + 1] - b->by_order_off[o];
+            if (Ao.n_sub) DIC_HIP_TRY(launch(o, Ao));
+        }
+    } else {
+        if (!h_list) {
+            set_error("launch_icgn_plain: a list over mixed warp orders needs its host copy");
+            return DIC_ERR_INVALID_INPUT;
+        }
+        for (const OrderRun& r : order_runs(b, h_list, count)) {
+            PlainArgs Ao = A;
+            Ao.list = list + r.start;
+            Ao.n_sub = r.count;
+            if (Ao.p0_list) Ao.p0_list = p0_list + 12 * r.start;
+            if (Ao.res_list) Ao.res_list = res_list + r.start;
+            DIC_HIP_TRY(launch(r.order, Ao));
+        }
+    }
     return DIC_OK;
 }
 
@@ -699,8 +736,8 @@ __global__ void __launch_bounds__(64) residual_kernel(ResidualArgs A) {
 
 }  // namespace
 
-int launch_residual(Batch* b, const Image* f, const Image* g, const int32_t* d_list, int64_t count, const double* d_p,
-                    bool p_by_subset, const int64_t* d_off, double* d_out) {
+int launch_residual(Batch* b, const Image* f, const Image* g, int order, const int32_t* d_list, int64_t count,
+                    const double* d_p, bool p_by_subset, const int64_t* d_off, double* d_out) {
     Engine* e = b->eng;
     ResidualArgs A;
     A.fC = f->coef64;
@@ -721,9 +758,9 @@ int launch_residual(Batch* b, const Image* f, const Image* g, const int32_t* d_l
     A.out = d_out;
     if (count == 0) return DIC_OK;
     const dim3 grid((unsigned)count), block(64);
-    if (b->max_order == 0)
+    if (order == 0)
         residual_kernel<2><<<grid, block, 0, e->stream>>>(A);
-    else if (b->max_order == 1)
+    else if (order == 1)
         residual_kernel<6><<<grid, block, 0, e->stream>>>(A);
     else
         residual_kernel<12><<<grid, block, 0, e->stream>>>(A);

@@ -83,7 +83,8 @@ struct Batch {
     int32_t n_tpl = 0;
     int64_t total_px = 0;     // Σ n_px over subsets (scratch sizing)
     int32_t max_n = 0;        // largest template
-    int32_t max_order = 0;
+    int32_t max_order = 0;    // highest warp order in the batch
+    bool mixed_orders = false;  // the subsets do not all share one warp order
     // device arrays
     double* d_coords = nullptr;     // n x 2
     int32_t* d_orders = nullptr;    // n
@@ -96,8 +97,9 @@ struct Batch {
     int32_t max_side = 0;           // largest template box side
     int32_t win_slack = 3;          // deformation slack of the LDS window (pixels per side)
     int32_t waves_per_subset = 0;   // batched engine: wavefronts per subset (0 = smallest that fits)
-    int32_t tuned_wps[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};    // autotuned wavefronts by a class's fewest (0 = none)
-    int32_t tuned_slack[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // autotuned wavefronts and slack by warp order and a class's fewest wavefronts (0 = none)
+    int32_t tuned_wps[3][9] = {};
+    int32_t tuned_slack[3][9] = {};
     int64_t* d_scr_off = nullptr;   // n: h_scr_off on the device
     double* d_p0 = nullptr;         // n x 12
     double* d_p = nullptr;          // n x 12 final warps
@@ -106,7 +108,11 @@ struct Batch {
     int32_t hist_cap = 0;
     double* d_scratch = nullptr;    // plain path per-pixel arrays
     size_t scratch_doubles = 0;
-    int32_t* d_order_perm = nullptr;  // dispatch order: template size descending, then nearest the image edge first
+    int32_t* d_order_perm = nullptr;  // dispatch order: warp order descending, template size descending, then
+                                      // nearest the image edge first
+    int32_t* d_by_order = nullptr;    // mixed orders: the subsets grouped by warp order, ascending index
+    int64_t by_order_off[4] = {0, 0, 0, 0};  // group o is d_by_order[by_order_off[o], by_order_off[o + 1])
+    std::vector<int32_t> h_orders;    // warp order per subset
     std::vector<double> h_coords;       // host copies for the dispatch order
     std::vector<int32_t> h_ext;
     std::vector<int32_t> h_perm;
@@ -143,14 +149,24 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
 // own p0 and results are then neither read nor written for them)
 // ext_scratch: a caller-provided per-pixel scratch of ext_doubles doubles
 // instead of the batch's own (grown on demand)
+// h_list: the same list as the host reads it; a list that mixes warp
+// orders must be grouped by order (each run is one launch)
 int launch_icgn_plain(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
-                      const int32_t* list = nullptr, int64_t count = 0, const double* p0_list = nullptr,
-                      dic_subset_result* res_list = nullptr, double* ext_scratch = nullptr, size_t ext_doubles = 0);
-// Subset::residual_map for the `count` listed subsets (device list), at the
-// warps d_p (12 per listed subset, or per subset of the batch when
-// p_by_subset), written to d_out from d_off[k] for listed subset k.
-int launch_residual(Batch* b, const Image* f, const Image* g, const int32_t* d_list, int64_t count, const double* d_p,
-                    bool p_by_subset, const int64_t* d_off, double* d_out);
+                      const int32_t* list = nullptr, const int32_t* h_list = nullptr, int64_t count = 0,
+                      const double* p0_list = nullptr, dic_subset_result* res_list = nullptr,
+                      double* ext_scratch = nullptr, size_t ext_doubles = 0);
+// Subset::residual_map for the `count` listed subsets (device list, all of
+// warp order `order`), at the warps d_p (12 per listed subset, or per
+// subset of the batch when p_by_subset), written to d_out from d_off[k] for
+// listed subset k.
+int launch_residual(Batch* b, const Image* f, const Image* g, int order, const int32_t* d_list, int64_t count,
+                    const double* d_p, bool p_by_subset, const int64_t* d_off, double* d_out);
+// Runs of equal warp order in the host list h[0, count): (start, length, order).
+struct OrderRun {
+    int64_t start, count;
+    int order;
+};
+std::vector<OrderRun> order_runs(const Batch* b, const int32_t* h, int64_t count);
 // per-pixel doubles of the plain kernel's scratch: f, grad f, g (+ grad g for FAGN)
 inline int plain_arrays(int method) { return method == DIC_METHOD_FAGN ? 6 : 4; }
 // The library's fixed device scratch for dic_batch_solve_noalloc: its
@@ -162,6 +178,7 @@ double* noalloc_scratch(int device);
 // icgn_batch.hip
 struct SizeClass {
     int64_t start, count;  // run of the dispatch order
+    int order;             // warp order of the class's subsets
     int base_wps;          // fewest wavefronts that hold the class's subsets
     int wps;               // wavefronts per subset used
     int slack;             // LDS window slack used

@@ -482,8 +482,9 @@ def solve_subsets(f_img, g_img, coords, local_mask, p0=None, subset_order=1, pre
     """Solve many subsets in one launch (Subset::new + solve per subset).
 
     local_mask is one Mask for every subset or a sequence of Masks, one per
-    subset (uneven radii).  precision "f64" is the plain FP64 drop-in, "f32"
-    the batched engine.  Returns a structured array of results (fields as
+    subset (uneven radii); subset_order is one warp order or one per subset
+    (orders may mix).  precision "f64" is the plain FP64 drop-in, "f32" the
+    batched engine.  Returns a structured array of results (fields as
     dic_subset_result)."""
     coords = np.asarray(coords, dtype=np.float64).reshape(-1, 2)
     if isinstance(local_mask, (list, tuple)):

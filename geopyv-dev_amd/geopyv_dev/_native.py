@@ -379,14 +379,18 @@ class DeviceImage:
 
 
 class Batch:
-    """A set of subsets solved together (dic_batch_*)."""
+    """A set of subsets solved together (dic_batch_*).  `order` is one warp
+    order for every subset, or one per subset (orders may mix)."""
 
     def __init__(self, engine, coords, order, tpl_index, tpl_start, tpl_xy):
         L = load()
         self.engine = engine
         coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 2)
         n = len(coords)
-        orders = np.full(n, int(order), dtype=np.int32)
+        if np.ndim(order) == 0:
+            orders = np.full(n, int(order), dtype=np.int32)
+        else:
+            orders = np.ascontiguousarray(order, dtype=np.int32).reshape(n)
         tpl_index = np.ascontiguousarray(tpl_index, dtype=np.int32)
         tpl_start = np.ascontiguousarray(tpl_start, dtype=np.int64)
         tpl_xy = np.ascontiguousarray(tpl_xy, dtype=np.int16).reshape(-1, 2)
@@ -395,7 +399,8 @@ class Batch:
                                  len(tpl_start) - 1, ptr(tpl_start), ptr(tpl_xy), ctypes.byref(h)))
         self.handle = h
         self.n = n
-        self.order = int(order)
+        self.orders = orders
+        self.order = int(orders.max())  # the highest warp order present
         self.tpl_index = tpl_index
         self.tpl_start = tpl_start
         self._keep = (coords, orders, tpl_xy)

@@ -149,7 +149,11 @@ int dic_mask_update(int shape, int size, double cx, double cy, const uint8_t* ma
  * Pixel lists are shared templates: template t holds the integer offsets
  * tpl_xy[2*tpl_start[t] .. 2*tpl_start[t+1]) as (dx, dy) pairs, in the
  * reference's row-major scan order; subset s uses template tpl_index[s]
- * about centre coords[2s], coords[2s+1] with warp order orders[s]. */
+ * about centre coords[2s], coords[2s+1] with warp order orders[s].  Warp
+ * orders may differ between subsets: a solve then launches each order's
+ * size classes side by side (the batched engine) or one launch per order
+ * (the FP64 kernel), and list solves and residual maps group their lists
+ * by order the same way; results are per subset either way. */
 int dic_batch_create(dic_engine* e, int64_t n_subsets, const double* coords, const int32_t* orders,
                      const int32_t* tpl_index, int32_t n_templates, const int64_t* tpl_start,
                      const int16_t* tpl_xy, dic_batch** out);
@@ -203,8 +207,9 @@ int dic_batch_autotune(dic_batch* b, const dic_image* f_img, const dic_image* g_
  * class of subsets needing 1, 2, 4 and 8 wavefronts (k = 0..3): the
  * autotuned choice if any, else the fewest wavefronts and the batch's
  * slack; with wavefronts set by dic_batch_set_schedule, that count for
- * every class.  No reference counterpart (the reference has no device
- * schedule). */
+ * every class.  A batch of mixed warp orders reports the classes of its
+ * highest order (each order is tuned on its own).  No reference
+ * counterpart (the reference has no device schedule). */
 int dic_batch_get_schedule(dic_batch* b, int32_t* wps, int32_t* slack);
 /* Solves only the listed subsets of a batch (each index at most once), each
  * from its row of p0 (count x 12, host; missing entries zero), and writes
