@@ -34,13 +34,6 @@ void dev_free(Engine* e, void* p, size_t bytes) {
     e->bytes_in_use -= (int64_t)bytes;
 }
 
-#define CHECK_ARG(cond, msg)                  \
-    do {                                      \
-        if (!(cond)) {                        \
-            set_error(std::string(msg));      \
-            return DIC_ERR_INVALID_INPUT;     \
-        }                                     \
-    } while (0)
 
 // Argument checks shared by the solve entry points.
 int validate_solve(const Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {

@@ -17,6 +17,15 @@ namespace geodic {
 // Thread-local message for dic_last_error().
 void set_error(const std::string& msg);
 
+// Argument check of an entry point: records msg and returns
+// DIC_ERR_INVALID_INPUT when cond fails.
+#define CHECK_ARG(cond, msg)                  \
+    do {                                      \
+        if (!(cond)) {                        \
+            ::geodic::set_error(std::string(msg)); \
+            return DIC_ERR_INVALID_INPUT;     \
+        }                                     \
+    } while (0)
 #define DIC_HIP_TRY(expr)                                                                   \
     do {                                                                                    \
         hipError_t _e = (expr);                                                             \

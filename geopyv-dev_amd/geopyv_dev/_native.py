@@ -26,6 +26,8 @@ ERR_SINGULAR_HESSIAN = 10
 ERR_SINGULAR_WARP = 11
 ERR_DEVICE = 20
 ERR_OUT_OF_MEMORY = 21
+ERR_BUSY = 22
+PENDING = 23
 
 PIX_U8, PIX_F64, PIX_RGB8_LOAD, PIX_GREY8_LOAD = 0, 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
@@ -46,6 +48,7 @@ EXPORTED = [
     "dic_image_stage", "dic_batch_reset_warps", "dic_frame_graph_create", "dic_frame_graph_launch",
     "dic_frame_graph_destroy", "dic_batch_solve_list", "dic_mesh_solve", "dic_mesh_element_strains",
     "dic_image_grey", "dic_batch_solve_noalloc", "dic_batch_get_schedule", "dic_batch_residual_map",
+    "dic_server_create", "dic_submit", "dic_poll", "dic_server_stats", "dic_server_destroy",
 ]
 
 
@@ -57,6 +60,23 @@ class SolveParams(ctypes.Structure):
         ("want_history", ctypes.c_int32),
         ("tolerance", ctypes.c_double),
         ("max_norm", ctypes.c_double),
+    ]
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("n_slots", ctypes.c_int32), ("max_queued", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("hbm_budget", ctypes.c_int64)]
+
+
+class JobStruct(ctypes.Structure):
+    """dic_job."""
+    _fields_ = [
+        ("ref_key", ctypes.c_uint64), ("ref_pixels", ctypes.c_void_p), ("tar_pixels", ctypes.c_void_p),
+        ("pix_type", ctypes.c_int32), ("border", ctypes.c_int32), ("h", ctypes.c_int64), ("w", ctypes.c_int64),
+        ("n_subsets", ctypes.c_int64), ("coords", ctypes.c_void_p), ("orders", ctypes.c_void_p),
+        ("tpl_index", ctypes.c_void_p), ("n_templates", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("tpl_start", ctypes.c_void_p), ("tpl_xy", ctypes.c_void_p), ("p0", ctypes.c_void_p),
+        ("params", SolveParams),
     ]
 
 
@@ -211,6 +231,12 @@ def load():
                                      ctypes.c_int32, _vp, _vp, ctypes.POINTER(MeshParams),
                                      ctypes.POINTER(Progress), ctypes.POINTER(MeshOutput)]
         L.dic_mesh_element_strains.argtypes = [ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp]
+        L.dic_server_create.argtypes = [ctypes.POINTER(ServerParams), ctypes.POINTER(_vp)]
+        L.dic_submit.argtypes = [_vp, ctypes.POINTER(JobStruct), ctypes.POINTER(ctypes.c_uint64)]
+        L.dic_poll.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int32, _vp, _dp]
+        L.dic_server_stats.argtypes = [_vp, _i64p, _i64p, _i64p, _i64p]
+        L.dic_server_destroy.argtypes = [_vp]
+        L.dic_server_destroy.restype = None
         _lib = L
         return L
 
@@ -374,6 +400,76 @@ class DeviceImage:
             if getattr(self, "handle", None):
                 load().dic_image_destroy(self.handle)
                 self.handle = None
+        except Exception:
+            pass
+
+
+class NativeServer:
+    """The library's serving loop (dic_server_create / dic_submit / dic_poll):
+    submit() queues a job and returns its ticket; poll() returns (records,
+    latency_ms) once it has finished, None while it runs (wait_ms 0)."""
+
+    def __init__(self, n_slots=4, hbm_budget=64 << 30, device=0, max_queued=0):
+        prm = ServerParams(int(device), int(n_slots), int(max_queued), 0, int(hbm_budget))
+        h = _vp()
+        check(load().dic_server_create(ctypes.byref(prm), ctypes.byref(h)))
+        self.handle = h
+        self._n = {}
+
+    def submit(self, ref_key, ref, tar, coords, orders, tpl_index, tpl_start, tpl_xy, p0=None, border=20,
+               method="icgn", precision=PREC_COMP_F32, max_iterations=50, tolerance=0.75, max_norm=1e-5):
+        ref = np.asarray(ref)
+        if ref.dtype == np.uint8:
+            pix = PIX_U8
+            ref = np.ascontiguousarray(ref)
+            tar = np.ascontiguousarray(tar, dtype=np.uint8)
+        else:
+            pix = PIX_F64
+            ref = np.ascontiguousarray(ref, dtype=np.float64)
+            tar = np.ascontiguousarray(tar, dtype=np.float64)
+        if tar.shape != ref.shape or ref.ndim != 2:
+            raise ValueError("reference and target must be 2-D images of one shape")
+        coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 2)
+        n = len(coords)
+        orders = (np.full(n, int(orders), np.int32) if np.ndim(orders) == 0
+                  else np.ascontiguousarray(orders, dtype=np.int32).reshape(n))
+        tpl_index = np.ascontiguousarray(tpl_index, dtype=np.int32).reshape(n)
+        tpl_start = np.ascontiguousarray(tpl_start, dtype=np.int64)
+        tpl_xy = np.ascontiguousarray(tpl_xy, dtype=np.int16).reshape(-1, 2)
+        p0a = None if p0 is None else np.ascontiguousarray(p0, dtype=np.float64).reshape(n, 12)
+        prm = SolveParams(METHOD[method], int(precision), int(max_iterations), 0, float(tolerance), float(max_norm))
+        j = JobStruct(int(ref_key), ref.ctypes.data, tar.ctypes.data, pix, int(border), ref.shape[0], ref.shape[1],
+                      n, coords.ctypes.data, orders.ctypes.data, tpl_index.ctypes.data, len(tpl_start) - 1, 0,
+                      tpl_start.ctypes.data, tpl_xy.ctypes.data, None if p0a is None else p0a.ctypes.data, prm)
+        t = ctypes.c_uint64()
+        check(load().dic_submit(self.handle, ctypes.byref(j), ctypes.byref(t)))  # copies the arrays
+        self._n[t.value] = n
+        return t.value
+
+    def poll(self, ticket, wait_ms=-1):
+        n = self._n[ticket]
+        out = np.zeros(n, dtype=RESULT_DTYPE)
+        lat = ctypes.c_double()
+        st = load().dic_poll(self.handle, ctypes.c_uint64(ticket), int(wait_ms), ptr(out), ctypes.byref(lat))
+        if st == PENDING:
+            return None
+        del self._n[ticket]
+        check(st)
+        return out, lat.value
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        check(load().dic_server_stats(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("queued", "running", "finished", "cache_bytes"), (x.value for x in v)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().dic_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

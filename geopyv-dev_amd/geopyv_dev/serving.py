@@ -1,15 +1,16 @@
 """Serving loop: continuous batching of correlation jobs on one GPU.
 
 Jobs (a reference/target image pair, subset centres with per-subset radii,
-warp order, initial warps) arrive over time.  The server admits a job as
-soon as a slot is free and the job's device-memory estimate fits the HBM
-budget beside the jobs in flight and the cached reference images; each
-admitted job runs on its slot's engine (its own HIP stream), so jobs that
-arrive while others are on the GPU overlap with them on the device.
-Reference B-spline coefficients stay resident in an LRU cache keyed by the
-job's reference key (inside the same budget, evicted only when unused), so
-a stream of jobs against one reference prefilters it once.  Latency is
-measured from a job's arrival to its results being on the host.
+warp order, initial warps) arrive over time.  `Server` hands each to the
+library's serving loop (dic_submit / dic_poll, csrc/serving.cpp): worker
+threads there, one engine and stream per slot, admit the queued jobs first
+come first served as soon as a slot is free and the job's device bytes fit
+the HBM budget beside the jobs in flight and the cached reference images,
+so jobs that arrive while others are on the GPU overlap with them on the
+device.  Reference B-spline coefficients stay resident in the library's
+cache keyed by the job's reference key (inside the same budget, evicted only
+when unused), so a stream of jobs against one reference prefilters it once.
+Latency runs from a job's arrival to its records being on the host.
 
 Results do not depend on what else runs: each job's kernels read only its
 own buffers and the shared, read-only reference coefficients.
@@ -66,139 +67,71 @@ class Job:
 
 class Server:
     """n_slots concurrent jobs, admitted first-come first-served within
-    hbm_budget bytes (jobs in flight plus cached references)."""
+    hbm_budget bytes (jobs in flight plus cached references), by the
+    library's serving loop; this class builds each job's templates,
+    submits it at its arrival and collects the records in submission
+    order."""
 
     def __init__(self, n_slots=4, hbm_budget=64 << 30, device=0):
         self.device = device
         self.budget = int(hbm_budget)
-        self.engines = [_native.Engine(device) for _ in range(n_slots)]
-        self.cache_engine = _native.Engine(device)
-        self.cache = collections.OrderedDict()  # key -> [DeviceImage, bytes, users]
-        self.cache_bytes = 0
-        self.in_flight = 0
-        self.pending = collections.deque()
-        self.cv = threading.Condition()
+        self.native = _native.NativeServer(n_slots=n_slots, hbm_budget=hbm_budget, device=device)
+        self.keys = {}        # reference key -> the library's 64-bit cache key
+        self.templates = {}   # radius -> circle template offsets
         self.results = {}
         self.errors = {}
-        self.reduce = None  # job, results, latency_ms -> what results[] keeps (all of it when None)
+        self.reduce = None    # job, results, latency_ms -> what results[] keeps (all of it when None)
         self.closed = False
-        self.free_slots = list(range(n_slots))
-        self.workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(n_slots)]
-        for t in self.workers:
-            t.start()
+        self.cv = threading.Condition()
+        self.tickets = collections.deque()
+        self.collector = threading.Thread(target=self._collect, daemon=True)
+        self.collector.start()
 
-    # ---- admission ----
+    def _template(self, r):
+        t = self.templates.get(r)
+        if t is None:
+            from . import Mask
+            t = np.rint(Mask(mask_type="local", shape="circle", size=int(r)).coords).astype(np.int16)
+            self.templates[r] = t
+        return t
+
+    # ---- admission: the library queues the job, this thread returns ----
     def submit(self, job, arrival=None):
+        job.arrival = time.perf_counter() if arrival is None else arrival
+        radii = np.unique(job.radii)
+        lists = [self._template(int(r)) for r in radii]
+        start = np.concatenate([[0], np.cumsum([len(a) for a in lists])]).astype(np.int64)
+        index = np.searchsorted(radii, job.radii).astype(np.int32)
+        key = self.keys.setdefault(job.ref_key, len(self.keys) + 1)
+        submitted = time.perf_counter()
+        ticket = self.native.submit(key, job.ref, job.tar, job.coords, job.order, index, start,
+                                    np.concatenate(lists), p0=job.p0, border=job.border,
+                                    precision=_native.PREC_COMP_F32, max_iterations=job.max_iterations,
+                                    tolerance=job.tolerance, max_norm=job.max_norm)
         with self.cv:
-            job.arrival = time.perf_counter() if arrival is None else arrival
-            self.pending.append(job)
+            self.tickets.append((job, ticket, submitted))
             self.cv.notify_all()
 
-    def _fits(self, job):
-        need = job.device_bytes() + (0 if job.ref_key in self.cache else job.image_bytes())
-        free = self.budget - self.in_flight - self.cache_bytes
-        if need <= free:
-            return True
-        # evict unused references, least recently used first
-        for key in list(self.cache):
-            if need <= free:
-                break
-            entry = self.cache[key]
-            if entry[2] == 0 and key != job.ref_key:
-                del self.cache[key]
-                self.cache_bytes -= entry[1]
-                free += entry[1]
-        return need <= free
-
-    def _next(self):
-        """Under the lock: the oldest pending job, if a slot is free and it
-        fits (first come, first served: no job overtakes the oldest).  A job
-        too large for the budget even alone runs when nothing else does."""
-        if not self.free_slots or not self.pending:
-            return None
-        job = self.pending[0]
-        if self._fits(job) or (self.in_flight == 0 and all(e[2] == 0 for e in self.cache.values())):
-            self.pending.popleft()
-            return job
-        return None
-
-    def _reference(self, job):
-        with self.cv:
-            entry = self.cache.get(job.ref_key)
-            if entry is not None:
-                self.cache.move_to_end(job.ref_key)
-                entry[2] += 1
-                return entry[0]
-        img = _native.DeviceImage(self.cache_engine, pixels=np.asarray(job.ref), border=job.border)
-        self.cache_engine.sync()
-        with self.cv:
-            entry = self.cache.get(job.ref_key)
-            if entry is None:
-                entry = [img, job.image_bytes(), 0]
-                self.cache[job.ref_key] = entry
-                self.cache_bytes += entry[1]
-            entry[2] += 1
-            return entry[0]
-
-    def _release(self, job):
-        with self.cv:
-            entry = self.cache.get(job.ref_key)
-            if entry is not None:
-                entry[2] -= 1
-
-    # ---- execution ----
-    def _worker(self):
+    def _collect(self):
         while True:
             with self.cv:
-                job = None
-                while job is None:
-                    if self.closed and not self.pending:
-                        return
-                    job = self._next()
-                    if job is None:
-                        self.cv.wait()
-                slot = self.free_slots.pop()
-                cost = job.device_bytes()
-                self.in_flight += cost
+                while not self.tickets and not self.closed:
+                    self.cv.wait()
+                if not self.tickets:
+                    return
+                job, ticket, submitted = self.tickets.popleft()
             try:
-                res = self._run(job, self.engines[slot])
-                done = time.perf_counter()
-                lat = (done - job.arrival) * 1e3
+                res, lat = self.native.poll(ticket, -1)
+                # the library times from dic_submit; add the wait from arrival
+                lat += (submitted - job.arrival) * 1e3
                 kept = (res, lat) if self.reduce is None else self.reduce(job, res, lat)
                 with self.cv:
                     self.results[job.job_id] = kept
+                    self.cv.notify_all()
             except Exception as exc:  # reported per job, the server keeps going
                 with self.cv:
                     self.errors[job.job_id] = exc
-            finally:
-                with self.cv:
-                    self.in_flight -= cost
-                    self.free_slots.append(slot)
                     self.cv.notify_all()
-
-    def _run(self, job, eng):
-        from . import Mask
-
-        f = self._reference(job)
-        try:
-            g = _native.DeviceImage(eng, pixels=np.asarray(job.tar), border=job.border)
-            # one circle template per distinct radius
-            radii = np.unique(job.radii)
-            lists = [np.rint(Mask(mask_type="local", shape="circle", size=int(r)).coords).astype(np.int16)
-                     for r in radii]
-            start = np.concatenate([[0], np.cumsum([len(a) for a in lists])]).astype(np.int64)
-            index = np.searchsorted(radii, job.radii).astype(np.int32)
-            b = _native.Batch(eng, job.coords, job.order, index, start, np.concatenate(lists))
-            if job.p0 is not None:
-                b.set_p0(job.p0)
-            b.solve(f, g, method="icgn", precision=_native.PREC_COMP_F32, max_iterations=job.max_iterations,
-                    tolerance=job.tolerance, max_norm=job.max_norm)
-            eng.sync()
-            out = b.results()
-            del b, g
-            return out
-        finally:
-            self._release(job)
 
     def run(self, jobs, arrivals=None):
         """Submit jobs at their arrival offsets (seconds from now; all at
@@ -262,5 +195,5 @@ class Server:
         with self.cv:
             self.closed = True
             self.cv.notify_all()
-        for t in self.workers:
-            t.join()
+        self.collector.join()
+        self.native.close()

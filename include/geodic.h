@@ -37,7 +37,8 @@ typedef enum {
     DIC_ERR_SINGULAR_WARP = 11,      /* panic "W_delta singular", subset.rs:520,595 */
     DIC_ERR_DEVICE = 20,             /* HIP runtime failure */
     DIC_ERR_OUT_OF_MEMORY = 21,      /* device allocation or HBM budget exceeded */
-    DIC_ERR_BUSY = 22                /* serving queue full */
+    DIC_ERR_BUSY = 22,               /* serving queue full */
+    DIC_PENDING = 23                 /* dic_poll: the job has not finished (not an error) */
 } dic_status;
 
 /* Pixel element types for image upload.  U8 and F64 are grey images taken
@@ -330,6 +331,65 @@ int dic_mesh_solve(dic_engine* e, const dic_image* f_img, const dic_image* g_img
  * (n_elements x 12) from node displacements (n_nodes x 2). */
 int dic_mesh_element_strains(int64_t n_nodes, const double* nodes, int64_t n_elements, int32_t mesh_order,
                              const int64_t* elements, const double* displacements, double* warps);
+
+/* ---- serving: continuous batching of correlation jobs on one GPU.  No
+ *      reference counterpart (geopyv-dev has no server); it is the submit /
+ *      poll path a serving caller binds instead of the Python loop ----
+ * dic_submit queues a job (a reference/target pair, its subsets and solve
+ * settings) and returns a ticket at once.  n_slots worker threads, each
+ * with its own engine and stream, admit the queued jobs first come first
+ * served as a slot frees and the job's device bytes fit within hbm_budget
+ * beside the jobs in flight and the cached references; a job too large for
+ * the budget alone runs when nothing else does.  Admitted jobs overlap on
+ * the device with those already running.  Reference images stay
+ * prefiltered on the device in a cache keyed by ref_key, evicted least
+ * recently used first and only when no job holds them.  A job's records
+ * do not depend on what else runs (its kernels read its own buffers and the
+ * read-only reference coefficients).  All calls are thread-safe. */
+typedef struct dic_server dic_server;
+typedef struct {
+    int32_t device;
+    int32_t n_slots;     /* jobs on the device at once, >= 1 */
+    int32_t max_queued;  /* waiting jobs beyond which dic_submit returns DIC_ERR_BUSY (0: no limit) */
+    int32_t reserved;    /* 0 */
+    int64_t hbm_budget;  /* device bytes for jobs in flight plus cached references */
+} dic_server_params;
+typedef struct {
+    uint64_t ref_key;        /* jobs with equal keys carry equal reference pixels */
+    const void* ref_pixels;  /* h x w host pixels of pix_type (DIC_PIX_U8 or DIC_PIX_F64) */
+    const void* tar_pixels;  /* the target, same shape and type */
+    int32_t pix_type;
+    int32_t border;          /* >= 3 (image.rs:52) */
+    int64_t h, w;
+    int64_t n_subsets;       /* subsets as for dic_batch_create */
+    const double* coords;
+    const int32_t* orders;
+    const int32_t* tpl_index;
+    int32_t n_templates;
+    int32_t reserved;        /* 0 */
+    const int64_t* tpl_start;
+    const int16_t* tpl_xy;
+    const double* p0;        /* n_subsets x 12 initial warps, or NULL for zero */
+    dic_solve_params params; /* want_history must be 0 */
+} dic_job;
+int dic_server_create(const dic_server_params* prm, dic_server** out);
+/* Copies the job's arrays (the caller may free them on return), queues it
+ * and sets *ticket.  DIC_ERR_BUSY when max_queued jobs are waiting. */
+int dic_submit(dic_server* s, const dic_job* job, uint64_t* ticket);
+/* Waits up to wait_ms (0: not at all, negative: until it finishes) for the
+ * job.  Finished: its n_subsets records go to out in subset order,
+ * *latency_ms (may be NULL) gets the time from dic_submit to the records
+ * on the host, the ticket is released and the job's status is returned
+ * (DIC_OK, or what its image, batch or solve call returned, with the
+ * message in dic_last_error).  Not finished: DIC_PENDING, out untouched.
+ * Unknown or released ticket: DIC_ERR_INVALID_INPUT. */
+int dic_poll(dic_server* s, uint64_t ticket, int32_t wait_ms, dic_subset_result* out, double* latency_ms);
+/* Jobs submitted, finished and not yet polled, and device bytes held by
+ * the cached references (any may be NULL). */
+int dic_server_stats(dic_server* s, int64_t* queued, int64_t* running, int64_t* finished, int64_t* cache_bytes);
+/* Finishes the queued and running jobs, stops the workers and frees the
+ * server (records not polled are dropped). */
+void dic_server_destroy(dic_server* s);
 
 #ifdef __cplusplus
 }

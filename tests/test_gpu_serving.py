@@ -121,3 +121,89 @@ def test_served_results_match_the_oracle():
               flush=True)
         assert agree >= 0.99 and both.mean() > 0.9
         assert du < 1e-4 and ds < 1e-5
+
+
+# ---- the library's serving loop through the C ABI (dic_server_create,
+# dic_submit, dic_poll, dic_server_stats, dic_server_destroy) ----
+
+def _abi_job(seed, shape=(512, 512), nsub=400):
+    from geopyv_dev import _native
+    import geopyv_dev as gp
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sp = workloads.Speckle(shape, int(0.012 * shape[0] * shape[1]), 2.0, seed)
+    ref = sp.reference(2.0)
+    field = workloads.Affine(((shape[1] - 1) / 2, (shape[0] - 1) / 2), 0.01, 0.004, -0.002, -0.006)
+    tar = sp.target(field, 2.0, rng=np.random.Generator(np.random.PCG64(seed + 1)))
+    coords = np.column_stack([rng.uniform(50, shape[1] - 51, nsub), rng.uniform(50, shape[0] - 51, nsub)])
+    radii = rng.integers(15, 31, nsub)
+    u = np.unique(radii)
+    lists = [np.rint(gp.Mask(mask_type="local", shape="circle", size=int(r)).coords).astype(np.int16) for r in u]
+    start = np.concatenate([[0], np.cumsum([len(a) for a in lists])]).astype(np.int64)
+    index = np.searchsorted(u, radii).astype(np.int32)
+    orders = rng.integers(1, 3, nsub).astype(np.int32)
+    p0 = np.zeros((nsub, 12))
+    p0[:, :2] = np.rint(field.warp(coords[:, 0], coords[:, 1])[:, :2])
+    return dict(ref=ref, tar=tar, coords=coords, orders=orders, index=index, start=start, xy=np.concatenate(lists),
+                p0=p0)
+
+
+def test_abi_server_records_equal_a_direct_solve():
+    from geopyv_dev import _native
+    import geopyv_dev as gp
+    jobs = [_abi_job(60 + k) for k in range(3)]
+    srv = _native.NativeServer(n_slots=2, hbm_budget=8 << 30)
+    tickets = [srv.submit(1 + k, j["ref"], j["tar"], j["coords"], j["orders"], j["index"], j["start"], j["xy"],
+                          p0=j["p0"]) for k, j in enumerate(jobs)]
+    got = [srv.poll(t, -1) for t in tickets]
+    st = srv.stats()
+    assert st["queued"] == 0 and st["running"] == 0 and st["finished"] == 0 and st["cache_bytes"] > 0
+    # a released ticket is refused
+    out1 = np.zeros(1, _native.RESULT_DTYPE)
+    assert _native.load().dic_poll(srv.handle, tickets[0], 0, _native.ptr(out1), None) == _native.ERR_INVALID_INPUT
+    srv.close()
+    eng = gp._engine()
+    for j, (rec, lat) in zip(jobs, got):
+        f = _native.DeviceImage(eng, pixels=j["ref"], border=20)
+        g = _native.DeviceImage(eng, pixels=j["tar"], border=20)
+        b = _native.Batch(eng, j["coords"], j["orders"], j["index"], j["start"], j["xy"])
+        b.set_p0(j["p0"])
+        b.solve(f, g, precision=_native.PREC_COMP_F32, max_norm=1e-5)
+        assert rec.tobytes() == b.results().tobytes()
+        assert lat > 0.0 and np.mean(rec["converged"]) > 0.9
+
+
+def test_abi_server_pending_busy_and_job_errors():
+    from geopyv_dev import _native
+    big = _abi_job(80, shape=(1024, 1024), nsub=3000)
+    small = _abi_job(81, nsub=50)
+    # one slot, at most one job waiting: while the large job runs, the first
+    # small job waits and a further one is refused (the large job's upload,
+    # prefilter and 3 000-subset solve take milliseconds; the submits, a
+    # copy of two small images each, far less)
+    srv = _native.NativeServer(n_slots=1, hbm_budget=8 << 30, max_queued=1)
+    sub = lambda j, key: srv.submit(key, j["ref"], j["tar"], j["coords"], j["orders"], j["index"], j["start"],
+                                    j["xy"], p0=j["p0"])
+    t_big = sub(big, 1)
+    t_small = sub(small, 2)
+    busy = None
+    try:
+        sub(small, 2)
+    except _native.GeodicError as e:
+        busy = e.status
+    first = srv.poll(t_big, 0)  # not waiting: the large job is most likely still running
+    rec_big, _ = srv.poll(t_big, -1) if first is None else first
+    rec_small, _ = srv.poll(t_small, -1)
+    print("busy status", busy, "pending on first poll", first is None)
+    assert busy in (None, _native.ERR_BUSY)
+    assert len(rec_big) == 3000 and len(rec_small) == 50
+    # a job the library refuses at run time (an invalid warp order) reports
+    # its status and message through dic_poll; the server keeps serving
+    bad = dict(small)
+    bad["orders"] = np.full(50, 5, np.int32)
+    t_bad = sub(bad, 2)
+    with pytest.raises(_native.GeodicError) as ei:
+        srv.poll(t_bad, -1)
+    assert ei.value.status == _native.ERR_INVALID_INPUT and "order" in str(ei.value)
+    again, _ = srv.poll(sub(small, 2), -1)
+    assert again.tobytes() == rec_small.tobytes()
+    srv.close()
