@@ -33,8 +33,12 @@
 //    are FP32; the partials are widened to FP64 for the wavefront DPP trees
 //    and the cross-wavefront sums (LDS, fixed wave order), so a sum's error
 //    is that of a dozen FP32 additions;
-//  * the Hessian is factorised once (FP64 Cholesky, reciprocal diagonal);
-//    each update Δp = -(LLᵀ)⁻¹∇ZNSSD and the composition run in FP64;
+//  * the Hessian is factorised and inverted once (FP64 Cholesky, one column
+//    of H⁻¹ per lane); for orders 0 and 1 the constant parts H⁻¹A and H⁻¹S
+//    of every update are formed then too, so each update
+//    Δp = -H⁻¹∇ZNSSD = -(H⁻¹A - (Δf/Δg)(H⁻¹B - (s1/n) H⁻¹S)) is one
+//    dot product and three operations per lane; it and the composition run
+//    in FP64;
 //  * the warp-adequacy diagnostic (subset.rs:748-820) needs only reference
 //    data except for the final residual e: its gradient standardisation and
 //    mode statistics are computed at setup, and the final pass that
@@ -493,6 +497,15 @@ __device__ __forceinline__ bool block_sum_lead(double* v, double* sRed, int lane
     return wave == 0;
 }
 
+// FP32 per-thread partials widened to FP64 and summed over the workgroup,
+// delivered to wavefront 0 only (returns true there), as block_sum_lead.
+template <int WPS, int V>
+__device__ __forceinline__ bool block_sum_lead32(const float* f, double* v, double* sRed, int lane, int wave) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[j] = (double)f[j];
+    return block_sum_lead<WPS, V>(v, sRed, lane, wave);
+}
+
 // FP32 per-thread partials widened to FP64 and summed over the workgroup.
 template <int WPS, int V>
 __device__ __forceinline__ void block_sum32(const float* f, double* v, double* sRed, int lane, int wave) {
@@ -652,12 +665,13 @@ __device__ __forceinline__ void chol_solve_neg_rd(const double* L, const double*
 
 // H⁻¹ of the symmetric positive definite Hessian given as its upper
 // triangle, row by row (hu[(a, b)], a <= b), through the Cholesky factor
-// L Lᵀ = H (one reciprocal per column, every step in registers) and the
-// two substitutions per column of the identity, each column stored (by the
-// lanes with `store`) as soon as it is solved; false when a pivot is not
-// positive (Hi is then meaningless).  Every lane computes the same values.
+// L Lᵀ = H (one reciprocal per column, every step in registers) and the two
+// substitutions of column c of the identity, run by lane c (lanes >= M
+// solve column 0 and store nothing), which stores column c of Hi; false
+// when a pivot is not positive (Hi is then meaningless).  Every lane
+// factorises alike, so a column's value does not depend on the lane.
 template <int M>
-__device__ __forceinline__ bool chol_inverse_regs(const double* hu, double* Hi, bool store) {
+__device__ __forceinline__ bool chol_inverse_cols(const double* hu, double* Hi, int lane) {
     double L[M][M], rd[M];
     bool ok = true;
 #pragma unroll
@@ -676,27 +690,25 @@ __device__ __forceinline__ bool chol_inverse_regs(const double* hu, double* Hi, 
             L[i][j] = t * rd[j];
         }
     }
+    const int c = lane < M ? lane : 0;
+    double y[M], x[M];
 #pragma unroll
-    for (int c = 0; c < M; ++c) {
-        double y[M], x[M];
+    for (int i = 0; i < M; ++i) {
+        double t = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-        for (int i = 0; i < M; ++i) {
-            double t = (i == c) ? 1.0 : 0.0;
+        for (int k = 0; k < i; ++k) t = fma(-L[i][k], y[k], t);
+        y[i] = t * rd[i];
+    }
 #pragma unroll
-            for (int k = 0; k < i; ++k) t = fma(-L[i][k], y[k], t);
-            y[i] = t * rd[i];
-        }
+    for (int i = M - 1; i >= 0; --i) {
+        double t = y[i];
 #pragma unroll
-        for (int i = M - 1; i >= 0; --i) {
-            double t = y[i];
+        for (int k = i + 1; k < M; ++k) t = fma(-L[k][i], x[k], t);
+        x[i] = t * rd[i];
+    }
+    if (lane < M) {
 #pragma unroll
-            for (int k = i + 1; k < M; ++k) t = fma(-L[k][i], x[k], t);
-            x[i] = t * rd[i];
-        }
-        if (store) {
-#pragma unroll
-            for (int i = 0; i < M; ++i) Hi[i * M + c] = x[i];
-        }
+        for (int i = 0; i < M; ++i) Hi[i * M + c] = x[i];
     }
     return ok;
 }
@@ -1039,6 +1051,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
     __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
     __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
+    __shared__ double sHAS[M <= 6 ? 2 * M : 1];  // H⁻¹A, H⁻¹S
     __shared__ double sG[3];  // g_m, Δg, squared norm of the last update
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[3];
@@ -1203,8 +1216,22 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             // Hessian (summed in the pass above) in registers, while the
             // other wavefronts go on to the second pass
             if (wave == 0) {
-                const bool ok = chol_inverse_regs<M>(w1 + 2 + 2 * M, sHv, lane == 0);
+                const bool ok = chol_inverse_cols<M>(w1 + 2 + 2 * M, sHv, lane);
                 if (lane == 0) sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_HESSIAN;
+                // row j of H⁻¹ times A and S, lane j: the constant parts of
+                // every update (A_j = Σ sdi_j (f - f_m), S_j = Σ sdi_j)
+                wave_lds_sync();
+                if (lane < M) {
+                    double ha = 0.0, hs = 0.0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        const double h = sHv[lane * M + k];
+                        ha = fma(h, w1[2 + k] - dlt * w1[2 + M + k], ha);
+                        hs = fma(h, w1[2 + M + k], hs);
+                    }
+                    sHAS[lane] = ha;
+                    sHAS[M + lane] = hs;
+                }
             }
         }
         if constexpr (M == 12) {
@@ -1358,9 +1385,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             if constexpr (kPredict)
                 pixel_pass<M, P, WPS, S, true>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, sDg[0], sDg[1],
                                                win, A);
-#pragma unroll
-            for (int j = 0; j < NV; ++j) sum[j] = (double)acc[j];
-            lead = block_sum_lead<WPS, NV>(sum, sRed, lane, wave);
+            lead = block_sum_lead32<WPS, NV>(acc, sum, sRed, lane, wave);
             // the projections' sums wait in LDS in case this pass is the last
             if (lead && lane == 0) {
 #pragma unroll
@@ -1369,9 +1394,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         } else {
             float acc[NA];
             pixel_pass<M, P, WPS, S, false>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, 0.f, 0.f, win, A);
-#pragma unroll
-            for (int j = 0; j < NA; ++j) sum[j] = (double)acc[j];
-            lead = block_sum_lead<WPS, NA>(sum, sRed, lane, wave);
+            lead = block_sum_lead32<WPS, NA>(acc, sum, sRed, lane, wave);
         }
         if (lead) {
             const double f_m = sK[0], delta_f = sK[1], sa = sK[2], rn = sK[3], size = sK[4];
@@ -1381,9 +1404,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             const double s1 = sum[0] + dn * shift;
             const double s2 = sum[1] + shift * (2.0 * sum[0] + dn * shift);
             const double s3 = sum[2] + shift * sa;
-            double Bj[M];
-#pragma unroll
-            for (int j = 0; j < M; ++j) Bj[j] = sum[3 + j] + shift * sAS[M + j];
             const double mg = s1 * rn;
             const double gm = f_m + mg;
             const double dg = sqrt(fmax(s2 - s1 * mg, 0.0));
@@ -1396,7 +1416,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 // and the whole wavefront composes it with the warp
                 double gz[M];
 #pragma unroll
-                for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
+                for (int j = 0; j < M; ++j)
+                    gz[j] = sAS[j] - ratio * (sum[3 + j] + shift * sAS[M + j] - mg * sAS[M + j]);
                 if (lane < M) {
                     double acc = 0.0;
 #pragma unroll
@@ -1409,15 +1430,17 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
                 nrm2 = convergence_norm2_dev<M>(sDp, size);
             } else {
-                double gz[M];
-#pragma unroll
-                for (int j = 0; j < M; ++j) gz[j] = sAS[j] - ratio * (Bj[j] - mg * sAS[M + j]);
                 // lane j: component j of -H⁻¹∇ZNSSD, then broadcast
                 double dp[M];
                 double acc = 0.0;
+                // with ∇ZNSSD = A - ratio (B' - mg S), B' = B + shift S:
+                // H⁻¹∇ZNSSD = H⁻¹A - ratio (H⁻¹B + (shift - mg) H⁻¹S)
                 if (lane < M) {
+                    double hb = 0.0;
 #pragma unroll
-                    for (int k = 0; k < M; ++k) acc = fma(sHv[lane * M + k], gz[k], acc);
+                    for (int k = 0; k < M; ++k) hb = fma(sHv[lane * M + k], sum[3 + k], hb);
+                    const double hs = sHAS[M + lane];
+                    acc = sHAS[lane] - ratio * fma(shift - mg, hs, hb);
                 }
 #pragma unroll
                 for (int k = 0; k < M; ++k) dp[k] = -readlane_f64(acc, k);
@@ -1510,9 +1533,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             float acc[NV];
             pixel_pass<M, P, WPS, S, true>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, sDg[0], sDg[1], win,
                                            A);
-#pragma unroll
-            for (int j = 0; j < NV; ++j) sum[j] = (double)acc[j];
-            lead = block_sum_lead<WPS, NV>(sum, sRed, lane, wave);
+            lead = block_sum_lead32<WPS, NV>(acc, sum, sRed, lane, wave);
         } else if (lead) {
 #pragma unroll
             for (int l = 0; l < 2 * K; ++l) sum[NA + l] = sFin[1 + l];
