@@ -20,7 +20,6 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <queue>
 #include <string>
 #include <thread>
 #include <utility>
@@ -32,11 +31,113 @@ namespace geodic {
 
 namespace {
 
-// Ascending neighbour lists of every node in one flat array.
+// Process-wide host workers for the traversal's larger per-node and
+// per-element passes (adjacency, the corrections' statistics, strains, the
+// initial warps of long lists).  run(n, grain, f) calls f(begin, end) on
+// chunks of [0, n) of `grain` items, taken in turn by the calling thread
+// and the workers, and returns when every chunk is done; a job of one
+// chunk runs on the caller.  Workers sleep between jobs (spinning ones
+// would take cycles from the thread that drives the device).  One job at a
+// time: a caller that finds another job running does its work alone.  The
+// job's fields are published seqlock-style: gen is odd while the next
+// caller rewrites them, which it does only once no worker is inside the
+// previous job, and a worker reads them only at an even gen.
+class HostPool {
+  public:
+    static HostPool& get() {
+        static HostPool p;
+        return p;
+    }
+    int threads() const { return (int)th_.size() + 1; }
+    template <class F>
+    void run(int64_t n, int64_t grain, F&& f) {
+        if (n <= 0) return;
+        grain = std::max<int64_t>(grain, 1);
+        std::unique_lock<std::mutex> job(run_mu_, std::try_to_lock);
+        if (n <= grain || th_.empty() || !job.owns_lock()) {
+            f((int64_t)0, n);
+            return;
+        }
+        gen_.fetch_add(1);  // odd: fields being rewritten
+        while (busy_.load() != 0) {
+        }
+        fn_ = [](void* c, int64_t b, int64_t e) { (*static_cast<F*>(c))(b, e); };
+        ctx_ = &f;
+        n_ = n;
+        grain_ = grain;
+        chunks_ = (n + grain - 1) / grain;
+        next_.store(0);
+        done_.store(0);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            gen_.fetch_add(1);  // even: published
+        }
+        cv_.notify_all();
+        work();
+        while (done_.load() < chunks_) {
+        }
+    }
+    ~HostPool() {
+        stop_.store(true);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            cv_.notify_all();
+        }
+        for (auto& t : th_) t.join();
+    }
+
+  private:
+    HostPool() {
+        const int hc = (int)std::thread::hardware_concurrency();  // 0 when unknown
+        const int nt = std::max(0, std::min(16, hc) - 1);
+        for (int k = 0; k < nt; ++k) th_.emplace_back([this] { loop(); });
+    }
+    void work() {
+        for (int64_t c; (c = next_.fetch_add(1)) < chunks_;) {
+            fn_(ctx_, c * grain_, std::min(n_, (c + 1) * grain_));
+            done_.fetch_add(1);
+        }
+    }
+    void loop() {
+        uint64_t seen = gen_.load();
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] {
+                    const uint64_t g = gen_.load();
+                    return stop_.load() || (g != seen && !(g & 1));
+                });
+            }
+            if (stop_.load()) return;
+            busy_.fetch_add(1);
+            const uint64_t g = gen_.load();
+            if (!(g & 1) && g != seen) {
+                work();
+                seen = g;
+            }
+            busy_.fetch_sub(1);
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_;
+    std::atomic<uint64_t> gen_{0};
+    alignas(64) std::atomic<int64_t> next_{0};
+    alignas(64) std::atomic<int64_t> done_{0};
+    alignas(64) std::atomic<int> busy_{0};
+    std::atomic<bool> stop_{false};
+    void (*fn_)(void*, int64_t, int64_t) = nullptr;
+    void* ctx_ = nullptr;
+    int64_t n_ = 0, grain_ = 1, chunks_ = 0;
+};
+
+// Ascending neighbour lists of every node in one flat array (node indices
+// fit in 32 bits: at most 2^30 nodes).
 struct Csr {
-    std::vector<int64_t> off, idx;
-    const int64_t* begin(int64_t i) const { return idx.data() + off[i]; }
-    const int64_t* end(int64_t i) const { return idx.data() + off[i + 1]; }
+    std::vector<int64_t> off;
+    std::vector<int32_t> idx;
+    const int32_t* begin(int64_t i) const { return idx.data() + off[i]; }
+    const int32_t* end(int64_t i) const { return idx.data() + off[i + 1]; }
 };
 
 struct Topology {
@@ -45,48 +146,68 @@ struct Topology {
     const int64_t* el = nullptr;
     Csr full;  // nodes sharing an element (connectivity, full = true; mesh.rs:1705-1764)
     Csr edge;  // edge-adjacent nodes (full = false): corner <-> midside for 6-node elements
+    std::vector<int64_t> inc_off, inc;  // elements incident to each node
     int64_t at(int64_t e, int k) const { return el[e * cols + k]; }
+    // for 3-node elements every pair of nodes in an element shares an edge
+    const Csr& edges() const { return order == 2 ? edge : full; }
 };
 
 // Adjacency::build (mesh.rs:1853-1868) as CSR: incident elements per node,
-// then each node's neighbours sorted and de-duplicated.
+// then each node's neighbours sorted and de-duplicated, the nodes in
+// parallel (a count pass, the offsets, a fill pass; a node's list does not
+// depend on the thread that builds it).
 void build_topology(Topology& t) {
-    std::vector<int64_t> cnt(t.n_nodes + 1, 0);
+    const int64_t n = t.n_nodes;
+    std::vector<int64_t>& cnt = t.inc_off;
+    cnt.assign(n + 1, 0);
     for (int64_t e = 0; e < t.n_el; ++e)
         for (int k = 0; k < t.cols; ++k) ++cnt[t.at(e, k) + 1];
-    for (int64_t i = 0; i < t.n_nodes; ++i) cnt[i + 1] += cnt[i];
-    std::vector<int64_t> inc(cnt[t.n_nodes]);
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    t.inc.resize(cnt[n]);
     std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
     for (int64_t e = 0; e < t.n_el; ++e)
-        for (int k = 0; k < t.cols; ++k) inc[fill[t.at(e, k)]++] = e;
+        for (int k = 0; k < t.cols; ++k) t.inc[fill[t.at(e, k)]++] = e;
     // the two edge partners of each position of a 6-node element
     // [c0, c1, c2, m01, m12, m20] (mesh.rs:1745-1752)
     static const int partner[6][2] = {{3, 5}, {3, 4}, {4, 5}, {0, 1}, {1, 2}, {0, 2}};
-    std::vector<int64_t> tmp;
+    HostPool& pool = HostPool::get();
+    constexpr int64_t kGrain = 2048;
     for (int pass = 0; pass < (t.order == 2 ? 2 : 1); ++pass) {
         Csr& out = pass == 0 ? t.full : t.edge;
-        out.off.assign(1, 0);
-        out.idx.clear();
-        for (int64_t i = 0; i < t.n_nodes; ++i) {
+        const auto gather = [&](int64_t i, std::vector<int32_t>& tmp) {
             tmp.clear();
             for (int64_t j = cnt[i]; j < cnt[i + 1]; ++j) {
-                const int64_t e = inc[j];
+                const int64_t e = t.inc[j];
                 for (int k = 0; k < t.cols; ++k) {
                     if (pass == 0) {
-                        if (t.at(e, k) != i) tmp.push_back(t.at(e, k));
+                        if (t.at(e, k) != i) tmp.push_back((int32_t)t.at(e, k));
                     } else if (t.at(e, k) == i) {
                         for (int a = 0; a < 2; ++a)
-                            if (t.at(e, partner[k][a]) != i) tmp.push_back(t.at(e, partner[k][a]));
+                            if (t.at(e, partner[k][a]) != i) tmp.push_back((int32_t)t.at(e, partner[k][a]));
                     }
                 }
             }
             std::sort(tmp.begin(), tmp.end());
             tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-            out.idx.insert(out.idx.end(), tmp.begin(), tmp.end());
-            out.off.push_back((int64_t)out.idx.size());
-        }
+        };
+        out.off.assign(n + 1, 0);
+        pool.run(n, kGrain, [&](int64_t b, int64_t e) {
+            std::vector<int32_t> tmp;
+            for (int64_t i = b; i < e; ++i) {
+                gather(i, tmp);
+                out.off[i + 1] = (int64_t)tmp.size();
+            }
+        });
+        for (int64_t i = 0; i < n; ++i) out.off[i + 1] += out.off[i];
+        out.idx.resize(out.off[n]);
+        pool.run(n, kGrain, [&](int64_t b, int64_t e) {
+            std::vector<int32_t> tmp;
+            for (int64_t i = b; i < e; ++i) {
+                gather(i, tmp);
+                std::copy(tmp.begin(), tmp.end(), out.idx.begin() + out.off[i]);
+            }
+        });
     }
-    if (t.order == 1) t.edge = t.full;
 }
 
 // lower and upper quartiles of the finite values: the percentile of
@@ -121,6 +242,49 @@ int64_t total_key(double a) {
     std::memcpy(&x, &a, 8);
     return x ^ (int64_t)((uint64_t)(x >> 63) >> 1);
 }
+
+// Max-heap of (key, node) pairs in std::pair order, four children per
+// entry (shallower than a binary heap, so a pop touches fewer cache lines).
+// The pairs are distinct, so the pop order is that of any max-heap.
+class MaxHeap {
+  public:
+    using Item = std::pair<uint64_t, int64_t>;
+    bool empty() const { return v_.empty(); }
+    const Item& top() const { return v_.front(); }
+    void push(const Item& x) {
+        size_t i = v_.size();
+        v_.push_back(x);
+        while (i > 0) {
+            const size_t p = (i - 1) / 4;
+            if (!(v_[p] < x)) break;
+            v_[i] = v_[p];
+            i = p;
+        }
+        v_[i] = x;
+    }
+    void pop() {
+        const Item x = v_.back();
+        v_.pop_back();
+        const size_t n = v_.size();
+        if (n == 0) return;
+        size_t i = 0;
+        for (;;) {
+            const size_t c0 = 4 * i + 1;
+            if (c0 >= n) break;
+            size_t best = c0;
+            const size_t c1 = std::min(c0 + 4, n);
+            for (size_t c = c0 + 1; c < c1; ++c)
+                if (v_[best] < v_[c]) best = c;
+            if (!(x < v_[best])) break;
+            v_[i] = v_[best];
+            i = best;
+        }
+        v_[i] = x;
+    }
+
+  private:
+    std::vector<Item> v_;
+};
 
 uint64_t heap_key(double c) {
     uint64_t k;
@@ -290,7 +454,8 @@ void initial_warp(const double* nodes, const double* tgt, const std::vector<int6
 
 // element_strains (mesh.rs:1552-1665): centroid displacement and the
 // first (and for 6-node elements second) displacement gradients from the
-// shape functions at the centroid.
+// shape functions at the centroid; elements on the host workers, the
+// first degenerate one (in element order) reported.
 int strains(const double* nodes, const Topology& t, const double* disp, double* warps, std::string& err) {
     static const double N1[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};
     static const double dN1[2][6] = {{1.0, 0.0, -1.0}, {0.0, 1.0, -1.0}};
@@ -302,124 +467,63 @@ int strains(const double* nodes, const Topology& t, const double* disp, double* 
     const int ns = t.order == 1 ? 3 : 6;
     const double* N = t.order == 1 ? N1 : N2;
     const double(*dN)[6] = t.order == 1 ? dN1 : dN2;
-    for (int64_t e = 0; e < t.n_el; ++e) {
-        double* w = warps + 12 * e;
-        double X[6][2], U[6][2];
-        for (int k = 0; k < ns; ++k) {
-            const int64_t i = t.at(e, k);
-            X[k][0] = nodes[2 * i];
-            X[k][1] = nodes[2 * i + 1];
-            U[k][0] = disp[2 * i];
-            U[k][1] = disp[2 * i + 1];
-        }
-        for (int k = 0; k < 12; ++k) w[k] = 0.0;
-        for (int k = 0; k < ns; ++k) w[0] += N[k] * U[k][0];
-        for (int k = 0; k < ns; ++k) w[1] += N[k] * U[k][1];
-        double jx[2][2] = {{0, 0}, {0, 0}}, ju[2][2] = {{0, 0}, {0, 0}};
-        for (int r = 0; r < 2; ++r)
-            for (int c = 0; c < 2; ++c)
-                for (int k = 0; k < ns; ++k) {
-                    jx[r][c] += dN[r][k] * X[k][c];
-                    ju[r][c] += dN[r][k] * U[k][c];
-                }
-        const double det = jx[0][0] * jx[1][1] - jx[0][1] * jx[1][0];
-        if (std::fabs(det) < 2.220446049250313e-16 * 1e6) {
-            err = "degenerate Jacobian at element " + std::to_string(e);
-            return DIC_ERR_MESH_GENERATION;
-        }
-        const double ij[2][2] = {{jx[1][1] / det, -jx[0][1] / det}, {-jx[1][0] / det, jx[0][0] / det}};
-        // [du/dx, dv/dx, du/dy, dv/dy]
-        for (int r = 0; r < 2; ++r)
-            for (int c = 0; c < 2; ++c) w[2 + 2 * r + c] = ij[r][0] * ju[0][c] + ij[r][1] * ju[1][c];
-        if (t.order == 2) {
-            double ku[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-            for (int r = 0; r < 3; ++r)
+    std::atomic<int64_t> first_bad{t.n_el};
+    HostPool::get().run(t.n_el, 2048, [&](int64_t e0, int64_t e1) {
+        for (int64_t e = e0; e < e1; ++e) {
+            double* w = warps + 12 * e;
+            double X[6][2], U[6][2];
+            for (int k = 0; k < ns; ++k) {
+                const int64_t i = t.at(e, k);
+                X[k][0] = nodes[2 * i];
+                X[k][1] = nodes[2 * i + 1];
+                U[k][0] = disp[2 * i];
+                U[k][1] = disp[2 * i + 1];
+            }
+            for (int k = 0; k < 12; ++k) w[k] = 0.0;
+            for (int k = 0; k < ns; ++k) w[0] += N[k] * U[k][0];
+            for (int k = 0; k < ns; ++k) w[1] += N[k] * U[k][1];
+            double jx[2][2] = {{0, 0}, {0, 0}}, ju[2][2] = {{0, 0}, {0, 0}};
+            for (int r = 0; r < 2; ++r)
                 for (int c = 0; c < 2; ++c)
-                    for (int k = 0; k < 6; ++k) ku[r][c] += d2N[r][k] * U[k][c];
-            // area coordinates' gradients of the corner triangle
-            const double x0 = X[0][0], y0 = X[0][1], x1 = X[1][0], y1 = X[1][1], x2 = X[2][0], y2 = X[2][1];
-            const double a2 = (x1 * y2 - x2 * y1) - (x0 * y2 - x2 * y0) + (x0 * y1 - x1 * y0);
-            const double g[2][2] = {{(y1 - y2) / a2, (y2 - y0) / a2}, {(x2 - x1) / a2, (x0 - x2) / a2}};
-            const double kx[3][3] = {{g[0][0] * g[0][0], 2.0 * g[0][0] * g[0][1], g[0][1] * g[0][1]},
-                                     {g[0][0] * g[1][0], g[0][0] * g[1][1] + g[0][1] * g[1][0], g[0][1] * g[1][1]},
-                                     {g[1][0] * g[1][0], 2.0 * g[1][0] * g[1][1], g[1][1] * g[1][1]}};
-            // [uxx, vxx, uxy, vxy, uyy, vyy]
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 2; ++c) w[6 + 2 * r + c] = kx[r][0] * ku[0][c] + kx[r][1] * ku[1][c] + kx[r][2] * ku[2][c];
+                    for (int k = 0; k < ns; ++k) {
+                        jx[r][c] += dN[r][k] * X[k][c];
+                        ju[r][c] += dN[r][k] * U[k][c];
+                    }
+            const double det = jx[0][0] * jx[1][1] - jx[0][1] * jx[1][0];
+            if (std::fabs(det) < 2.220446049250313e-16 * 1e6) {
+                int64_t cur = first_bad.load();
+                while (e < cur && !first_bad.compare_exchange_weak(cur, e)) {
+                }
+                continue;
+            }
+            const double ij[2][2] = {{jx[1][1] / det, -jx[0][1] / det}, {-jx[1][0] / det, jx[0][0] / det}};
+            // [du/dx, dv/dx, du/dy, dv/dy]
+            for (int r = 0; r < 2; ++r)
+                for (int c = 0; c < 2; ++c) w[2 + 2 * r + c] = ij[r][0] * ju[0][c] + ij[r][1] * ju[1][c];
+            if (t.order == 2) {
+                double ku[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 2; ++c)
+                        for (int k = 0; k < 6; ++k) ku[r][c] += d2N[r][k] * U[k][c];
+                // area coordinates' gradients of the corner triangle
+                const double x0 = X[0][0], y0 = X[0][1], x1 = X[1][0], y1 = X[1][1], x2 = X[2][0], y2 = X[2][1];
+                const double a2 = (x1 * y2 - x2 * y1) - (x0 * y2 - x2 * y0) + (x0 * y1 - x1 * y0);
+                const double g[2][2] = {{(y1 - y2) / a2, (y2 - y0) / a2}, {(x2 - x1) / a2, (x0 - x2) / a2}};
+                const double kx[3][3] = {{g[0][0] * g[0][0], 2.0 * g[0][0] * g[0][1], g[0][1] * g[0][1]},
+                                         {g[0][0] * g[1][0], g[0][0] * g[1][1] + g[0][1] * g[1][0], g[0][1] * g[1][1]},
+                                         {g[1][0] * g[1][0], 2.0 * g[1][0] * g[1][1], g[1][1] * g[1][1]}};
+                // [uxx, vxx, uxy, vxy, uyy, vyy]
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 2; ++c) w[6 + 2 * r + c] = kx[r][0] * ku[0][c] + kx[r][1] * ku[1][c] + kx[r][2] * ku[2][c];
+            }
         }
+    });
+    if (first_bad.load() < t.n_el) {
+        err = "degenerate Jacobian at element " + std::to_string(first_bad.load());
+        return DIC_ERR_MESH_GENERATION;
     }
     return DIC_OK;
 }
-
-// Host workers for the per-node preparation of large lists (the initial
-// warps of a colour read only solved nodes, so its nodes are independent).
-// Created on first use; the calling thread works too.
-class Workers {
-  public:
-    ~Workers() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : th_) t.join();
-    }
-    template <class F>
-    void run(int64_t n, F&& f) {
-        if (n < 256) {
-            for (int64_t i = 0; i < n; ++i) f(i);
-            return;
-        }
-        if (th_.empty()) start();
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            job_ = [&f](int64_t i) { f(i); };
-            n_ = n;
-            next_ = 0;
-            busy_ = (int)th_.size();
-            ++gen_;
-        }
-        cv_.notify_all();
-        work();
-        std::unique_lock<std::mutex> lk(mu_);
-        done_.wait(lk, [&] { return busy_ == 0; });
-        job_ = nullptr;
-    }
-
-  private:
-    void start() {
-        const int hc = (int)std::thread::hardware_concurrency();  // 0 when unknown
-        const int nt = std::max(1, std::min(16, hc) - 1);
-        for (int k = 0; k < nt; ++k) th_.emplace_back([this] { loop(); });
-    }
-    void loop() {
-        uint64_t seen = 0;
-        for (;;) {
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-            }
-            work();
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--busy_ == 0) done_.notify_one();
-        }
-    }
-    void work() {
-        for (int64_t i; (i = next_.fetch_add(32)) < n_;)
-            for (int64_t j = i; j < std::min(i + 32, n_); ++j) job_(j);
-    }
-    std::vector<std::thread> th_;
-    std::mutex mu_;
-    std::condition_variable cv_, done_;
-    std::function<void(int64_t)> job_;
-    int64_t n_ = 0;
-    std::atomic<int64_t> next_{0};
-    int busy_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-};
 
 // The traversal's per-node state (the arrays of solve_impl, mesh.rs:639-648).
 struct Run {
@@ -433,7 +537,7 @@ struct Run {
     std::vector<uint8_t> stored, propagated, qok;
     std::vector<double> czn, P, disp, norms;
     std::vector<int32_t> iters;
-    std::priority_queue<std::pair<uint64_t, int64_t>> heap;
+    MaxHeap heap;
     int64_t n_solves = 0, n_lists = 0;
     double list_seconds = 0.0;  // wall time inside list solves (launches, device work, synchronisation)
     // list-solve buffers
@@ -442,7 +546,6 @@ struct Run {
     std::vector<dic_subset_result> ra, rb;
     std::vector<int64_t> retry;
     std::vector<int32_t> cand_pos;  // node -> its index among the current candidates, else -1
-    Workers workers;
     const dic_progress* pg = nullptr;  // observer (may be null)
 
     bool cancelled() const { return pg && pg->cancelled && pg->cancelled(pg->user) != 0; }
@@ -461,9 +564,20 @@ struct Run {
     const std::vector<int64_t>& trusted(int64_t i, const uint8_t* skip) const {
         thread_local std::vector<int64_t> out;
         out.clear();
-        for (const int64_t* q = t->full.begin(i); q != t->full.end(i); ++q)
+        for (const int32_t* q = t->full.begin(i); q != t->full.end(i); ++q)
             if (qok[*q] && (skip ? !skip[*q] : stored[*q] != 0)) out.push_back(*q);
         return out;
+    }
+
+    // precondition_warp of every listed node (each reads only solved nodes,
+    // so they are independent), on the host workers for longer lists
+    void warps(const std::vector<int64_t>& members, const uint8_t* skip, std::vector<double>& pa) const {
+        const int64_t n = (int64_t)members.size();
+        HostPool::get().run(n, n < 256 ? n : 64, [&](int64_t b, int64_t e) {
+            for (int64_t k = b; k < e; ++k)
+                initial_warp(nodes, &nodes[2 * members[k]], trusted(members[k], skip), czn.data(), P.data(), plen,
+                             &pa[k * plen]);
+        });
     }
 
     int list(const std::vector<int64_t>& nodes_in, const double* warps, double tol, std::vector<dic_subset_result>& r) {
@@ -556,7 +670,7 @@ struct Run {
 
     // solve_neighbours_from (mesh.rs:1112-1152)
     int rg_from(int64_t cur) {
-        for (const int64_t* q = t->edge.begin(cur); q != t->edge.end(cur); ++q) {
+        for (const int32_t* q = t->edges().begin(cur); q != t->edges().end(cur); ++q) {
             const int64_t nb = *q;
             if (stored[nb]) continue;
             dic_subset_result r;
@@ -577,7 +691,7 @@ struct Run {
         if (cand_pos.size() != (size_t)t->n_nodes) cand_pos.assign(t->n_nodes, -1);
         std::vector<int64_t> cand;
         for (int64_t r : roots)
-            for (const int64_t* q = t->edge.begin(r); q != t->edge.end(r); ++q)
+            for (const int32_t* q = t->edges().begin(r); q != t->edges().end(r); ++q)
                 if (!stored[*q] && cand_pos[*q] == -1) {
                     cand_pos[*q] = -2;
                     cand.push_back(*q);
@@ -589,13 +703,26 @@ struct Run {
         std::vector<uint8_t> used;
         for (size_t k = 0; k < cand.size(); ++k) cand_pos[cand[k]] = (int32_t)k;
         for (size_t k = 0; k < cand.size(); ++k) {
-            used.assign(n_col + 1, 0);
-            for (const int64_t* q = t->full.begin(cand[k]); q != t->full.end(cand[k]); ++q) {
+            // lowest colour no coloured neighbour has: a bit mask while the
+            // colours fit in 64, else a flag per colour
+            uint64_t mask = 0;
+            bool wide = n_col > 63;
+            if (wide) used.assign(n_col + 1, 0);
+            for (const int32_t* q = t->full.begin(cand[k]); q != t->full.end(cand[k]); ++q) {
                 const int32_t j = cand_pos[*q];
-                if (j >= 0 && colour[j] >= 0) used[colour[j]] = 1;
+                if (j >= 0 && colour[j] >= 0) {
+                    if (wide)
+                        used[colour[j]] = 1;
+                    else
+                        mask |= (uint64_t)1 << colour[j];
+                }
             }
             int32_t c = 0;
-            while (used[c]) ++c;
+            if (wide) {
+                while (used[c]) ++c;
+            } else {
+                c = __builtin_ctzll(~mask);  // n_col <= 63 keeps a zero bit
+            }
             colour[k] = c;
             n_col = std::max(n_col, c + 1);
         }
@@ -608,10 +735,7 @@ struct Run {
             for (size_t k = 0; k < cand.size(); ++k)
                 if (colour[k] == c) members.push_back(cand[k]);
             pa.assign(members.size() * plen, 0.0);
-            workers.run((int64_t)members.size(), [&](int64_t k) {
-                initial_warp(nodes, &nodes[2 * members[k]], trusted(members[k], nullptr), czn.data(), P.data(), plen,
-                             &pa[k * plen]);
-            });
+            warps(members, nullptr, pa);
             int st = solve_nodes(members, pa, sp.tolerance, kept);
             if (!st) st = fold(members, kept);
             if (st) return st;
@@ -627,7 +751,7 @@ struct Run {
         if (!std::isfinite(d0) || !std::isfinite(d1)) return -1.0;
         double sx = 0.0, sy = 0.0;
         int64_t n = 0;
-        for (const int64_t* q = t->full.begin(i); q != t->full.end(i); ++q) {
+        for (const int32_t* q = t->full.begin(i); q != t->full.end(i); ++q) {
             const int64_t nb = *q;
             if ((exclude && exclude[nb]) || !std::isfinite(disp[2 * nb]) || !std::isfinite(disp[2 * nb + 1])) continue;
             sx += disp[2 * nb];
@@ -659,7 +783,7 @@ struct Run {
         int32_t n_waves = 0;
         for (size_t i = 0; i < seq.size(); ++i) {
             int32_t w = 0;
-            for (const int64_t* q = t->full.begin(seq[i]); q != t->full.end(seq[i]); ++q)
+            for (const int32_t* q = t->full.begin(seq[i]); q != t->full.end(seq[i]); ++q)
                 if (pos[*q] >= 0 && pos[*q] < (int64_t)i) w = std::max(w, wave[pos[*q]] + 1);
             wave[i] = w;
             n_waves = std::max(n_waves, w + 1);
@@ -678,10 +802,7 @@ struct Run {
             for (size_t i = 0; i < seq.size(); ++i)
                 if (wave[i] == w) members.push_back(seq[i]);
             pa.assign(members.size() * plen, 0.0);
-            workers.run((int64_t)members.size(), [&](int64_t k) {
-                initial_warp(nodes, &nodes[2 * members[k]], trusted(members[k], skip.data()), czn.data(), P.data(),
-                             plen, &pa[k * plen]);
-            });
+            warps(members, skip.data(), pa);
             const int st = solve_nodes(members, pa, sp.tolerance, kept);
             if (st) return st;
             for (size_t k = 0; k < members.size(); ++k) settle(members[k], kept[k]);
@@ -718,8 +839,14 @@ struct Run {
         // lane 2: flow outliers (below LQ - 1.5 IQR of the mesh) and
         // magnitude outliers (above UQ + 4 IQR of an element's
         // neighbourhood), mildest magnitude first, ties by node
+        HostPool& pool = HostPool::get();
         std::vector<double> fl(n), r(n);
-        for (int64_t i = 0; i < n; ++i) fl[i] = flow(i, nullptr);
+        pool.run(n, 4096, [&](int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; ++i) {
+                fl[i] = flow(i, nullptr);
+                r[i] = std::sqrt(disp[2 * i] * disp[2 * i] + disp[2 * i + 1] * disp[2 * i + 1]);
+            }
+        });
         double flq, fuq;
         {
             std::vector<double> v(fl);
@@ -727,44 +854,30 @@ struct Run {
         }
         const double fiqr = fuq - flq;
         std::vector<uint8_t> rflag(n, 0), flagged(n, 0);
-        for (int64_t i = 0; i < n; ++i) {
-            r[i] = std::sqrt(disp[2 * i] * disp[2 * i] + disp[2 * i + 1] * disp[2 * i + 1]);
+        for (int64_t i = 0; i < n; ++i)
             if (fl[i] < flq - 1.5 * fiqr) flagged[i] = 1;
-        }
-        // per element (independent of each other): host threads over
-        // element ranges, each marking its own flags, merged afterwards
-        const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(
-            std::min<int64_t>(16, (int64_t)std::thread::hardware_concurrency()), t->n_el / 2048));
-        std::vector<std::vector<uint8_t>> part(nth, std::vector<uint8_t>(nth > 1 ? n : 0, 0));
-        auto elems = [&](int w) {
-            std::vector<uint8_t>& mark = nth > 1 ? part[w] : rflag;
-            std::vector<int64_t> local;
+        // per element (independent of each other, on the host workers): the
+        // nodes of its neighbourhood above its fence are flagged; a node
+        // flagged by several elements gets the same mark from each
+        pool.run(t->n_el, 1024, [&](int64_t b, int64_t e1) {
+            std::vector<int32_t> local;
             std::vector<double> lr;
-            for (int64_t e = t->n_el * w / nth; e < t->n_el * (w + 1) / nth; ++e) {
+            for (int64_t e = b; e < e1; ++e) {
                 local.clear();
                 for (int k = 0; k < 3; ++k)
                     local.insert(local.end(), t->full.begin(t->at(e, k)), t->full.end(t->at(e, k)));
                 std::sort(local.begin(), local.end());
                 local.erase(std::unique(local.begin(), local.end()), local.end());
                 lr.clear();
-                for (int64_t q : local) lr.push_back(r[q]);
+                for (int32_t q : local) lr.push_back(r[q]);
                 double lq, uq;
                 quartiles(lr, lq, uq);
                 if (lr.empty()) continue;
                 const double fence = uq + 4.0 * (uq - lq);
-                for (int64_t q : local)
-                    if (r[q] > fence) mark[q] = 1;
+                for (int32_t q : local)
+                    if (r[q] > fence) __atomic_store_n(&rflag[q], (uint8_t)1, __ATOMIC_RELAXED);
             }
-        };
-        if (nth == 1) {
-            elems(0);
-        } else {
-            std::vector<std::thread> pool;
-            for (int w = 0; w < nth; ++w) pool.emplace_back(elems, w);
-            for (auto& th : pool) th.join();
-            for (int w = 0; w < nth; ++w)
-                for (int64_t i = 0; i < n; ++i) rflag[i] |= part[w][i];
-        }
+        });
         std::vector<int64_t> ids;
         for (int64_t i = 0; i < n; ++i)
             if (flagged[i] || rflag[i]) {
@@ -867,18 +980,27 @@ int dic_mesh_solve(dic_engine* d, const dic_image* f, const dic_image* g, int64_
     }
     const int plen = 6 * prm->subset_order;
     std::vector<int32_t> orders(n_nodes, prm->subset_order);
-    dic_batch* db = nullptr;
-    st = dic_batch_create(d, n_nodes, nodes, orders.data(), tpl_index, n_templates, tpl_start, tpl_xy, &db);
-    if (st) return st;
-    Batch* b = &db->b;
-    Engine* e = &d->e;
     Topology t;
     t.n_nodes = n_nodes;
     t.n_el = n_elements;
     t.order = mesh_order;
     t.cols = mesh_order == 1 ? 3 : 6;
     t.el = elements;
-    build_topology(t);
+    // the adjacency is built on a helper thread while the batch is created
+    // and the seed node solved, which do not read it; joined before the
+    // propagation (and on every way out)
+    std::thread topo([&t] { build_topology(t); });
+    struct Join {
+        std::thread& th;
+        ~Join() {
+            if (th.joinable()) th.join();
+        }
+    } join_topo{topo};
+    dic_batch* db = nullptr;
+    st = dic_batch_create(d, n_nodes, nodes, orders.data(), tpl_index, n_templates, tpl_start, tpl_xy, &db);
+    if (st) return st;
+    Batch* b = &db->b;
+    Engine* e = &d->e;
 
     Run R;
     R.t = &t;
@@ -923,6 +1045,7 @@ int dic_mesh_solve(dic_engine* d, const dic_image* f, const dic_image* g, int64_
             st = R.solve_nodes({seed}, pa, prm->seed_tolerance, kept);
         }
         if (!st) st = R.fold({seed}, kept);
+        topo.join();
         if (!st) {
             R.propagated[seed] = 1;
             if (prm->preconditioning == DIC_PRECOND_RG) {
