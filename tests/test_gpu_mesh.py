@@ -198,7 +198,7 @@ def test_speckle_layer_rg_f32_engine_matches_oracle(speckle):
     st_o, ro, st_g, msg, rg = _both(speckle, [256.0, 256.0], precision=_native.PREC_COMP_F32,
                                     preconditioning="layer-RG", workers=16, tolerance=0.75, seed_tolerance=0.9)
     assert st_o == 0 and st_g == 0, msg
-    _compare(ro, rg, disp_tol=1e-4, strain_tol=1e-5, zncc_tol=1e-5, iter_frac=0.8)
+    _compare(ro, rg, disp_tol=1e-4, strain_tol=1e-6, zncc_tol=1e-5, iter_frac=0.8)
 
 
 def test_order2_quadratic_mesh_matches_oracle(speckle):

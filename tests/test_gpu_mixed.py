@@ -4,7 +4,7 @@ launch side by side (the batched engine) or one launch per order (the FP64
 drop-in), list solves and residual maps group their lists by order.  Bars:
 the FP64 drop-in against the oracle within 1e-4 px and 1e-6 in the warp's
 gradient terms with identical statuses, convergence flags and iteration
-counts; the FP32 engine within 1e-4 px and 1e-5 in strain with convergence
+counts; the FP32 engine within 1e-4 px and 1e-6 in strain with convergence
 agreeing on at least 98 % of the subsets; a subset's record in the mixed
 batch equal bit for bit to its record in a batch of its own order alone."""
 
@@ -91,7 +91,7 @@ def test_mixed_orders_f32_engine_matches_oracle(mixed):
     agree = np.mean(r["converged"] == orc["converged"])
     print(f"mixed orders, f32 engine: max |du| {du:.2e} px, max |dstrain| {strain:.2e}, "
           f"convergence agreement {agree:.4f}")
-    assert du < 1e-4 and strain < 1e-5 and agree >= 0.98
+    assert du < 1e-4 and strain < 1e-6 and agree >= 0.98  # strain measured 6e-8
     np.testing.assert_allclose(r["c_zncc"][both], orc["c_zncc"][both], atol=1e-5)
     np.testing.assert_allclose(r["eta_u"][both], orc["eta_u"][both], rtol=2e-2, atol=1e-6)
 

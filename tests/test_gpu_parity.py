@@ -190,8 +190,10 @@ def test_batch_f32_engine_matches_oracle(mesh_small):
     it_mismatch = np.mean(r["iterations"] != orc["iterations"])
     print("f32 engine: max |du|", du, "max |dstrain|", ds, "iteration mismatch", it_mismatch,
           "conv", r["converged"].mean(), orc["converged"].mean())
+    # the displacement bar of the drop-in; the strain bar of the drop-in too
+    # (measured 9e-8 here, 2e-7 on the 40 000-subset headline pair)
     assert du < 1e-4
-    assert ds < 1e-5
+    assert ds < 1e-6
     assert np.array_equal(r["converged"], orc["converged"])
     np.testing.assert_allclose(r["c_zncc"], orc["c_zncc"], atol=1e-6)
 
@@ -256,7 +258,7 @@ def test_uneven_order2_f32_engine_matches_oracle(band_small):
           "converged", orc["converged"].mean())
     assert agree >= 0.98
     assert du < 1e-4
-    assert ds < 1e-5
+    assert ds < 1e-6  # measured 1.0e-7
     np.testing.assert_allclose(r["c_zncc"][both], orc["c_zncc"][both], atol=1e-5)
     np.testing.assert_allclose(r["eta_u"][both], orc["eta_u"][both], rtol=2e-2, atol=1e-6)
 

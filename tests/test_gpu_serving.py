@@ -120,7 +120,7 @@ def test_served_results_match_the_oracle():
               f"|dstrain| {ds:.2e}, convergence agreement {agree:.4f}, converged {orc['converged'].mean():.4f}",
               flush=True)
         assert agree >= 0.99 and both.mean() > 0.9
-        assert du < 1e-4 and ds < 1e-5
+        assert du < 1e-4 and ds < 1e-6  # strain measured at most 1.5e-7
 
 
 # ---- the library's serving loop through the C ABI (dic_server_create,
