@@ -332,6 +332,49 @@ __device__ __forceinline__ void gather_block(const float* win, int idx, int ls, 
     }
 }
 
+// The 5x5 block at the start of a 6x6 one (rows and columns 0-4), every
+// read issued before any use, with a compile-time row stride S (rows 4 on
+// from blk4 = blk + 4 S) or the run-time stride ls.
+template <int S>
+__device__ __forceinline__ void gather_block5(const float* win, int idx, int ls, float c[25]) {
+    const float* blk = win + idx;
+    if constexpr (S > 0) {
+        const float* blk4 = blk + 4 * ls;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) c[5 * r + k] = blk[r * S + k];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) c[20 + k] = blk4[k];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) c[5 * r + k] = blk[r * ls + k];
+    }
+    GEODIC_SCHED_BARRIER();
+}
+
+// Value, ∂x and ∂y at a knot (zero fraction on both axes), from the 5x5
+// block of its cell: there the value basis is (1/120, 13/60, 11/20, 13/60,
+// 1/120, 0), symmetric, and the derivative basis (-1/24, -5/12, 0, 5/12,
+// 1/24, 0), antisymmetric (the Q matrix's first two rows, image.rs:183-190,
+// at t = 0), so each row adds or subtracts its mirror pairs first.
+__device__ __forceinline__ float knot_grad(const float c[25], float& gx, float& gy) {
+    constexpr float b0 = (float)GQ00, b1 = (float)GQ01, b2 = (float)GQ02;
+    constexpr float d0 = (float)GQ14, d1 = (float)GQ13;
+    float s[5], t[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const float* q = c + 5 * r;
+        s[r] = fmaf(b0, q[0] + q[4], fmaf(b1, q[1] + q[3], b2 * q[2]));
+        t[r] = fmaf(d0, q[4] - q[0], d1 * (q[3] - q[1]));
+    }
+    gx = fmaf(b0, t[0] + t[4], fmaf(b1, t[1] + t[3], b2 * t[2]));
+    gy = fmaf(d0, s[4] - s[0], d1 * (s[3] - s[1]));
+    return fmaf(b0, s[0] + s[4], fmaf(b1, s[1] + s[3], b2 * s[2]));
+}
+
 __device__ __forceinline__ float contract6r(const float c[36], const float a[6], const float b[6]) {
     float val = 0.f;
 #pragma unroll
@@ -1139,18 +1182,22 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
 #pragma unroll
         for (int j = 0; j < V1; ++j) u[j] = 0.f;
         // every pixel sits at an integer offset from the centre: in the fast
-        // path one basis serves the whole subset
+        // path one basis serves the whole subset, and a centre on a whole
+        // pixel puts every pixel on a knot, where the 5x5 form applies
+        const bool knot = fxc == 0.f && fyc == 0.f;
         float ba[6], bb[6], bda[6], bdb[6];
-        basis_f32(fyc, ba);
-        basis_f32(fxc, bb);
-        dbasis_f32(fyc, bda);
-        dbasis_f32(fxc, bdb);
+        if (!knot) {
+            basis_f32(fyc, ba);
+            basis_f32(fxc, bb);
+            dbasis_f32(fyc, bda);
+            dbasis_f32(fxc, bdb);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            ba[q] = uni(ba[q]);
-            bb[q] = uni(bb[q]);
-            bda[q] = uni(bda[q]);
-            bdb[q] = uni(bdb[q]);
+            for (int q = 0; q < 6; ++q) {
+                ba[q] = uni(ba[q]);
+                bb[q] = uni(bb[q]);
+                bda[q] = uni(bda[q]);
+                bdb[q] = uni(bdb[q]);
+            }
         }
         const int LS = S > 0 ? S : W.ls;
         const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
@@ -1161,7 +1208,11 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             float v = 0.f, gx = 0.f, gy = 0.f;
             if (i < n) {
                 const float2 d = off_xy(sXY[i]);
-                if (fast) {
+                if (fast && knot) {
+                    float cc[25];
+                    gather_block5<S>(win, cb + (int)fmaf(d.y, wwf, d.x), W.ls, cc);
+                    v = knot_grad(cc, gx, gy);
+                } else if (fast) {
                     float cc[36];
                     gather_block<S>(win, cb + (int)fmaf(d.y, wwf, d.x), W.ls, cc);
                     v = contract6r_grad(cc, ba, bb, bda, bdb, gx, gy);
