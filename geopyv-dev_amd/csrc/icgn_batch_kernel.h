@@ -640,29 +640,31 @@ __device__ __forceinline__ void warp_pos(const float* q, float c0x, float c0y, f
     }
 }
 
-// Per-iteration FP32 accumulators: s1, s2, s3, B_0..B_{M-1}.
-template <int M>
+// Per-iteration FP32 accumulators: s1, s2, s3, B_0..B_{M-1}; without S3
+// the ZNSSD's s3 is left out and B_j follow s2.
+template <int M, bool S3 = true>
 __device__ __forceinline__ void accumulate(float* acc, float gp, float a, float gx, float gy, float dx, float dy) {
+    constexpr int B = S3 ? 3 : 2;  // B_j at acc[B + j]
     acc[0] += gp;
     acc[1] = fmaf(gp, gp, acc[1]);
-    acc[2] = fmaf(a, gp, acc[2]);
+    if constexpr (S3) acc[2] = fmaf(a, gp, acc[2]);
     const float tx = gx * gp, ty = gy * gp;
-    acc[3] += tx;
-    acc[4] += ty;
+    acc[B] += tx;
+    acc[B + 1] += ty;
     if (M >= 6) {
-        acc[5] = fmaf(tx, dx, acc[5]);
-        acc[6] = fmaf(ty, dx, acc[6]);
-        acc[7] = fmaf(tx, dy, acc[7]);
-        acc[8] = fmaf(ty, dy, acc[8]);
+        acc[B + 2] = fmaf(tx, dx, acc[B + 2]);
+        acc[B + 3] = fmaf(ty, dx, acc[B + 3]);
+        acc[B + 4] = fmaf(tx, dy, acc[B + 4]);
+        acc[B + 5] = fmaf(ty, dy, acc[B + 5]);
     }
     if (M == 12) {
         const float hx = 0.5f * dx * dx, hy = 0.5f * dy * dy, xy = dx * dy;
-        acc[9] = fmaf(tx, hx, acc[9]);
-        acc[10] = fmaf(ty, hx, acc[10]);
-        acc[11] = fmaf(tx, xy, acc[11]);
-        acc[12] = fmaf(ty, xy, acc[12]);
-        acc[13] = fmaf(tx, hy, acc[13]);
-        acc[14] = fmaf(ty, hy, acc[14]);
+        acc[B + 6] = fmaf(tx, hx, acc[B + 6]);
+        acc[B + 7] = fmaf(ty, hx, acc[B + 7]);
+        acc[B + 8] = fmaf(tx, xy, acc[B + 8]);
+        acc[B + 9] = fmaf(ty, xy, acc[B + 9]);
+        acc[B + 10] = fmaf(tx, hy, acc[B + 10]);
+        acc[B + 11] = fmaf(ty, hy, acc[B + 11]);
     }
 }
 
@@ -948,11 +950,12 @@ __device__ __forceinline__ Plan load_plan(const Plan* sp) {
 }
 
 // One pass over the pixels at plan `pl` (its window W staged less f_m):
-// acc[0, 3 + M) gets s1, s2, s3, B_j, and with DIAG acc[3 + M, 3 + M + 2K)
-// gets the diagnostic's projection sums Σ c·mode·g' per gradient axis
-// (c = ∇f less its mean (mx, my), g' = g - f_m).  The two forms perform
-// the same FP32 operations on the iteration sums.
-template <int M, int P, int WPS, int S, bool DIAG>
+// acc[0, 3 + M) gets s1, s2, s3, B_j (without S3: s1, s2, B_j in
+// acc[0, 2 + M)), and with DIAG acc[3 + M, 3 + M + 2K) gets the
+// diagnostic's projection sums Σ c·mode·g' per gradient axis (c = ∇f less
+// its mean (mx, my), g' = g - f_m).  Every form performs the same FP32
+// operations on the sums it shares with the others.
+template <int M, int P, int WPS, int S, bool DIAG, bool S3 = true>
 __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Window& W, const float (&am)[P],
                                            const float (&gxr)[P], const float (&gyr)[P], const OffXY* sXY, int n,
                                            int tid, int bx, int by, float f_m32, float mx, float my,
@@ -960,7 +963,8 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
     constexpr int NT = 64 * WPS;
     constexpr int PU = P <= 4 ? P : 1;
     constexpr int K = Modes<M>::K;
-    constexpr int NA = 3 + M;
+    static_assert(S3 || !DIAG, "a diagnostic pass sums s3");
+    constexpr int NA = (S3 ? 3 : 2) + M;
     constexpr int NV = DIAG ? NA + 2 * K : NA;
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = 0.f;
@@ -999,7 +1003,7 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
                 basis_f32_s(oy - ry, a);
                 basis_f32_s(ox - rx, b);
                 const float gp = contract6r(cc, a, b);
-                accumulate<M>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
+                accumulate<M, S3>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
                 diag(gp, d, k);
             }
         }
@@ -1017,11 +1021,18 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
                 cell_frac(by, oy, A.h, cy, ty);
                 float d0, d1;
                 const float gp = eval32<false>(win, A.gC, A.pitch, W, A.border, cx, cy, tx, ty, f_m32, d0, d1);
-                accumulate<M>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
+                accumulate<M, S3>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
                 diag(gp, d, k);
             }
         }
     }
+}
+
+// ZNSSD from the pass sums (subset.rs:418-446): 2 - 2 (s3 - mg Σa)/(Δf Δg),
+// contraction off so that every call site rounds alike.
+__device__ __forceinline__ double znssd_sums(double s3, double mg, double sa, double delta_f, double dg) {
+#pragma clang fp contract(off)
+    return 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
 }
 
 // The warp-adequacy scores (subset.rs:748-820) from the projection sums
@@ -1079,6 +1090,13 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
 #define GEODIC_PREDICT 1
 #endif
     constexpr bool kPredict = GEODIC_PREDICT && M <= 6;
+    // With the prediction, a pass not predicted to be the last leaves out
+    // s3 (it serves only the last iteration's ZNSSD and the history): one
+    // sum fewer per pixel and per tree.  History runs every pass as a
+    // predicted one; a missed prediction's ZNSSD comes from the pass after
+    // the loop, which evaluates the same warp with the same operations.
+    constexpr bool kS3 = !kPredict;
+    constexpr int NA0 = (kS3 ? 3 : 2) + M;  // sums of a pass not predicted to be the last
     constexpr double kDiagT2 = 36.0;
     GEODIC_DYN_LDS(float, lds);
     float* win = lds;
@@ -1095,7 +1113,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
     __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
     __shared__ double sHAS[M <= 6 ? 2 * M : 1];  // H⁻¹A, H⁻¹S
-    __shared__ double sG[3];  // g_m, Δg, squared norm of the last update
+    __shared__ double sG[4];  // g_m, Δg, squared norm of the last update, its mean s1/n
     __shared__ double sRed[WPS * kRedSlots];
     __shared__ int sCtl[3];
     __shared__ Plan sPlan;
@@ -1310,7 +1328,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
 #pragma unroll
             for (int k = 0; k < M; ++k) p0[k] = readlane_f64(p0v, k);
             Plan pl = make_plan<M, S>(p0, ext, xc, yc, fxc, fyc, no_window(), A);
-            pl.diag = (kPredict && A.max_it == 1) ? 1 : 0;
+            pl.diag = (kPredict && (A.max_it == 1 || A.hist)) ? 1 : 0;
             if (lane == 0) {
                 sCtl[2] = 0;
                 sPlan = pl;
@@ -1443,10 +1461,18 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 for (int l = 0; l < 2 * K; ++l) sFin[1 + l] = sum[NA + l];
             }
         } else {
-            float acc[NA];
-            pixel_pass<M, P, WPS, S, false>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, 0.f, 0.f, win, A);
-            lead = block_sum_lead32<WPS, NA>(acc, sum, sRed, lane, wave);
+            float acc[NA0];
+            pixel_pass<M, P, WPS, S, false, kS3>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, 0.f, 0.f, win,
+                                                  A);
+            lead = block_sum_lead32<WPS, NA0>(acc, sum, sRed, lane, wave);
+            if constexpr (!kS3) {
+                // the layout of a pass with s3 (none here)
+#pragma unroll
+                for (int j = M - 1; j >= 0; --j) sum[3 + j] = sum[2 + j];
+                sum[2] = 0.0;
+            }
         }
+        const bool has_s3 = kS3 || dg_it;
         if (lead) {
             const double f_m = sK[0], delta_f = sK[1], sa = sK[2], rn = sK[3], size = sK[4];
             const double max_norm2 = A.max_norm * A.max_norm;
@@ -1454,7 +1480,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             const double shift = (double)f_m32 - f_m;
             const double s1 = sum[0] + dn * shift;
             const double s2 = sum[1] + shift * (2.0 * sum[0] + dn * shift);
-            const double s3 = sum[2] + shift * sa;
+            const double s3 = fma(shift, sa, sum[2]);
             const double mg = s1 * rn;
             const double gm = f_m + mg;
             const double dg = sqrt(fmax(s2 - s1 * mg, 0.0));
@@ -1508,11 +1534,12 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             const bool conv = ok && nrm2 < max_norm2;
             const bool last = ok && (conv || it == A.max_it);
             const bool keep_h = ok && A.hist && it <= A.hist_cap;
-            // ZNSSD at this iteration's warp (subset.rs:418-446 in sums)
+            // ZNSSD at this iteration's warp (subset.rs:418-446 in sums); a
+            // last pass without s3 leaves it to the pass after the loop
             double znssd = 0.0;
-            if (last || keep_h) znssd = 2.0 - 2.0 * (s3 - mg * sa) / (delta_f * dg);
+            if ((last || keep_h) && has_s3) znssd = znssd_sums(s3, mg, sa, delta_f, dg);
             if (last && lane == 0) {
-                sFin[0] = znssd;
+                if (has_s3) sFin[0] = znssd;
                 if (dg_it) sCtl[2] = 1;  // its projections' sums are in sFin
             }
             if (keep_h && lane == 0) {
@@ -1529,7 +1556,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
 #pragma unroll
                 for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
                 nx = make_plan<M, S>(pn, ext, xc, yc, fxc, fyc, W, A);
-                nx.diag = (kPredict && (nrm2 < kDiagT2 * max_norm2 || it + 1 == A.max_it)) ? 1 : 0;
+                nx.diag = (kPredict && (A.hist || nrm2 < kDiagT2 * max_norm2 || it + 1 == A.max_it)) ? 1 : 0;
             }
             if (lane == 0) {
 #pragma unroll
@@ -1539,6 +1566,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 sG[0] = gm;
                 sG[1] = dg;
                 sG[2] = nrm2;
+                sG[3] = mg;
                 sPlan = nx;
             }
         }
@@ -1585,6 +1613,14 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             pixel_pass<M, P, WPS, S, true>(acc, pl, W, am, gxr, gyr, sXY, n, tid, bx, by, f_m32, sDg[0], sDg[1], win,
                                            A);
             lead = block_sum_lead32<WPS, NV>(acc, sum, sRed, lane, wave);
+            if constexpr (!kS3) {
+                // the last iteration's pass left out s3: its ZNSSD from this
+                // one, with that iteration's mean and spread
+                if (lead && lane == 0) {
+                    const double shift = (double)f_m32 - f_m;
+                    sFin[0] = znssd_sums(fma(shift, sa, sum[2]), sG[3], sa, delta_f, sG[1]);
+                }
+            }
         } else if (lead) {
 #pragma unroll
             for (int l = 0; l < 2 * K; ++l) sum[NA + l] = sFin[1 + l];
