@@ -197,13 +197,19 @@ __device__ __forceinline__ double dpp_move(double v) {
     return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double wave_sum_u(double v) {
+// The single-value tree: lane 63 ends with the total.
+__device__ __forceinline__ double wave_tree1(double v) {
     v += dpp_move<0xb1, 0xf, true>(v);   // quad_perm [1,0,3,2]
     v += dpp_move<0x4e, 0xf, true>(v);   // quad_perm [2,3,0,1]
     v += dpp_move<0x114, 0xf, true>(v);  // row_shr:4
     v += dpp_move<0x118, 0xf, true>(v);  // row_shr:8
     v += dpp_move<0x142, 0xa, false>(v); // row_bcast:15 into rows 1 and 3
     v += dpp_move<0x143, 0xc, false>(v); // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_u(double v) {
+    v = wave_tree1(v);
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
@@ -241,7 +247,9 @@ __device__ __forceinline__ double swap16_sum(double x, double y) {
     return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
-__device__ __forceinline__ void wave_sums8(const double* x, double* out) {
+// The tree of wave_sums8 up to its last step: the register whose lane
+// 16·row(o) + 8h + 4 holds value 4h + o.
+__device__ __forceinline__ double wave_tree8(const double* x) {
     const double r0 = swap32_sum(x[0], x[1]), r1 = swap32_sum(x[2], x[3]);
     const double r2 = swap32_sum(x[4], x[5]), r3 = swap32_sum(x[6], x[7]);
     const double t0 = swap16_sum(r0, r1), t1 = swap16_sum(r2, r3);
@@ -250,6 +258,11 @@ __device__ __forceinline__ void wave_sums8(const double* x, double* out) {
     w += dpp_move<0x114, 0xf, true>(w);  // row_shr:4 (lanes 4-7, 12-15 pair correctly)
     w += dpp_move<0x4e, 0xf, true>(w);   // quad_perm [2,3,0,1]
     w += dpp_move<0xb1, 0xf, true>(w);   // quad_perm [1,0,3,2]
+    return w;
+}
+
+__device__ __forceinline__ void wave_sums8(const double* x, double* out) {
+    const double w = wave_tree8(x);
     constexpr int kRow[4] = {0, 2, 1, 3};
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -257,6 +270,35 @@ __device__ __forceinline__ void wave_sums8(const double* x, double* out) {
         const int lo = __builtin_amdgcn_readlane(__double2loint(w), lane);
         const int hi = __builtin_amdgcn_readlane(__double2hiint(w), lane);
         out[j] = __hiloint2double(hi, lo);
+    }
+}
+
+// The V wavefront sums of wave_sums (the same trees, so the same values),
+// each written to dst[j] by the lane that holds it instead of being read
+// into scalar registers.  Needs every lane active.
+template <int V>
+__device__ __forceinline__ void wave_sums_to(const double* in, double* dst, int lane) {
+    constexpr int G = V / 8, R = V % 8;
+    const bool holder = (lane & 7) == 4;
+    const int row = lane >> 4;
+    const int slot = 4 * ((lane >> 3) & 1) + (row == 1 ? 2 : (row == 2 ? 1 : row));  // 4h + o
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const double w = wave_tree8(in + 8 * u);
+        if (holder) dst[8 * u + slot] = w;
+    }
+    if constexpr (R >= 3) {
+        double x[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = j < R ? in[8 * G + j] : 0.0;
+        const double w = wave_tree8(x);
+        if (holder && slot < R) dst[8 * G + slot] = w;
+    } else {
+#pragma unroll
+        for (int j = 8 * G; j < V; ++j) {
+            const double t = wave_tree1(in[j]);
+            if (lane == 63) dst[j] = t;
+        }
     }
 }
 

@@ -457,6 +457,14 @@ __device__ __forceinline__ float eval32(const float* win, const float* __restric
     return contract6(blk, (int)pitch, a, b) - shift;
 }
 
+// LDS writes of some lanes made visible to the other lanes of the same
+// wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // The wavefront partials of V values (rows of sRed, in wave order) summed
 // in wave order by lane j for value j, then broadcast from those lanes: WPS
 // LDS reads per lane instead of WPS x V, the same additions in the same
@@ -471,13 +479,41 @@ __device__ __forceinline__ void combine_lanes(const double* sRed, int lane, doub
     for (int k = 0; k < V; ++k) v[k] = readlane_f64(t, k);
 }
 
+// Rows of kRedSlots doubles in the reduction buffer: one per wavefront for
+// its partials, and with two wavefronts two more for the totals.
+template <int WPS>
+constexpr int red_rows() {
+    return WPS == 2 ? 4 : WPS;
+}
+
 // Sums V FP64 values over the workgroup: transposed DPP/permlane trees per
 // wavefront, then the wavefront partials in wave order through LDS (with
-// four or more wavefronts, one value per lane).  Result uniform (SGPRs) in
-// every wave.
+// four or more wavefronts, one value per lane).  With one or two
+// wavefronts each tree's results are written by the lanes that hold them,
+// lane j adds value j over the wavefronts (the same additions in the same
+// order) into its wavefront's totals row, and every lane reads the totals
+// back as lane-invariant loads; otherwise the results are read into scalar
+// registers.  Every wave gets the same values.
 template <int WPS, int V>
 __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int wave) {
     static_assert(V <= kRedSlots, "reduction buffer");
+    if constexpr (WPS == 1) {
+        wave_sums_to<V>(v, sRed, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = sRed[j];
+        return;
+    } else if constexpr (WPS == 2) {
+        wave_sums_to<V>(v, sRed + wave * kRedSlots, lane);
+        __syncthreads();
+        double* tot = sRed + (2 + wave) * kRedSlots;
+        if (lane < V) tot[lane] = sRed[lane] + sRed[kRedSlots + lane];
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = tot[j];
+        __syncthreads();  // both have read the partials before either writes again
+        return;
+    }
     double w[V];
     wave_sums<V>(v, w);
     if constexpr (WPS > 1) {
@@ -506,9 +542,29 @@ __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int
 
 // The same sums delivered to wavefront 0 only (the one that then runs the
 // serial update); returns true there.  Same addition order as block_sum.
+// The partials rows are written again only after the caller's next
+// workgroup barrier.
 template <int WPS, int V>
 __device__ __forceinline__ bool block_sum_lead(double* v, double* sRed, int lane, int wave) {
     static_assert(V <= kRedSlots, "reduction buffer");
+    if constexpr (WPS == 1) {
+        wave_sums_to<V>(v, sRed, lane);
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = sRed[j];
+        return true;
+    } else if constexpr (WPS == 2) {
+        wave_sums_to<V>(v, sRed + wave * kRedSlots, lane);
+        __syncthreads();
+        if (wave == 0) {
+            double* tot = sRed + 2 * kRedSlots;
+            if (lane < V) tot[lane] = sRed[lane] + sRed[kRedSlots + lane];
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < V; ++j) v[j] = tot[j];
+        }
+        return wave == 0;
+    }
     double w[V];
     wave_sums<V>(v, w);
     if constexpr (WPS >= 4) {
@@ -758,13 +814,6 @@ __device__ __forceinline__ bool chol_inverse_cols(const double* hu, double* Hi, 
     return ok;
 }
 
-// LDS writes of some lanes made visible to the other lanes of the same
-// wavefront.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 // The factorisation of cholesky<M> (L packed row-major lower triangle, rd
 // = reciprocal diagonal) by the lanes of one wavefront, column by column:
@@ -1114,7 +1163,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
     __shared__ double sHAS[M <= 6 ? 2 * M : 1];  // H⁻¹A, H⁻¹S
     __shared__ double sG[4];  // g_m, Δg, squared norm of the last update, its mean s1/n
-    __shared__ double sRed[WPS * kRedSlots];
+    __shared__ double sRed[red_rows<WPS>() * kRedSlots];
     __shared__ int sCtl[3];
     __shared__ Plan sPlan;
     const int tid = threadIdx.x;
