@@ -247,8 +247,8 @@ __device__ __forceinline__ double swap16_sum(double x, double y) {
     return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
-// The tree of wave_sums8 up to its last step: the register whose lane
-// 16·row(o) + 8h + 4 holds value 4h + o.
+// Eight wavefront sums at once: the register whose lane 16·row(o) + 8h + 4
+// holds the sum of value 4h + o, row = (0, 2, 1, 3)[o].
 __device__ __forceinline__ double wave_tree8(const double* x) {
     const double r0 = swap32_sum(x[0], x[1]), r1 = swap32_sum(x[2], x[3]);
     const double r2 = swap32_sum(x[4], x[5]), r3 = swap32_sum(x[6], x[7]);
@@ -261,21 +261,10 @@ __device__ __forceinline__ double wave_tree8(const double* x) {
     return w;
 }
 
-__device__ __forceinline__ void wave_sums8(const double* x, double* out) {
-    const double w = wave_tree8(x);
-    constexpr int kRow[4] = {0, 2, 1, 3};
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int lane = 16 * kRow[j & 3] + 8 * (j >> 2) + 4;
-        const int lo = __builtin_amdgcn_readlane(__double2loint(w), lane);
-        const int hi = __builtin_amdgcn_readlane(__double2hiint(w), lane);
-        out[j] = __hiloint2double(hi, lo);
-    }
-}
-
-// The V wavefront sums of wave_sums (the same trees, so the same values),
-// each written to dst[j] by the lane that holds it instead of being read
-// into scalar registers.  Needs every lane active.
+// The V wavefront sums of in[], each written to dst[j] by the lane that
+// holds it (no lane reads into scalar registers): groups of eight through
+// wave_tree8, a remainder of three or more as a padded group, else
+// single-value trees.  Needs every lane active.
 template <int V>
 __device__ __forceinline__ void wave_sums_to(const double* in, double* dst, int lane) {
     constexpr int G = V / 8, R = V % 8;
@@ -302,24 +291,6 @@ __device__ __forceinline__ void wave_sums_to(const double* in, double* dst, int 
     }
 }
 
-template <int V>
-__device__ __forceinline__ void wave_sums(const double* in, double* out) {
-    constexpr int G = V / 8, R = V % 8;
-#pragma unroll
-    for (int u = 0; u < G; ++u) wave_sums8(in + 8 * u, out + 8 * u);
-    if constexpr (R >= 3) {
-        // a padded group is cheaper than three or more single trees
-        double x[8], o[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) x[j] = j < R ? in[8 * G + j] : 0.0;
-        wave_sums8(x, o);
-#pragma unroll
-        for (int j = 0; j < R; ++j) out[8 * G + j] = o[j];
-    } else {
-#pragma unroll
-        for (int j = 8 * G; j < V; ++j) out[j] = wave_sum_u(in[j]);
-    }
-}
 
 // A template offset pair as stored in LDS: two half-precision numbers
 // (offsets are integers far inside the exact range of FP16), so the FP32

@@ -465,78 +465,47 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// The wavefront partials of V values (rows of sRed, in wave order) summed
-// in wave order by lane j for value j, then broadcast from those lanes: WPS
-// LDS reads per lane instead of WPS x V, the same additions in the same
-// order as summing every value in every lane.
-template <int WPS, int V>
-__device__ __forceinline__ void combine_lanes(const double* sRed, int lane, double* v) {
-    const int j = lane < V ? lane : V - 1;
-    double t = sRed[j];
-#pragma unroll
-    for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
-#pragma unroll
-    for (int k = 0; k < V; ++k) v[k] = readlane_f64(t, k);
-}
-
 // Rows of kRedSlots doubles in the reduction buffer: one per wavefront for
-// its partials, and with two wavefronts two more for the totals.
+// its partials and, with several wavefronts, one per wavefront for totals.
 template <int WPS>
 constexpr int red_rows() {
-    return WPS == 2 ? 4 : WPS;
+    return WPS == 1 ? 1 : 2 * WPS;
 }
 
-// Sums V FP64 values over the workgroup: transposed DPP/permlane trees per
-// wavefront, then the wavefront partials in wave order through LDS (with
-// four or more wavefronts, one value per lane).  With one or two
-// wavefronts each tree's results are written by the lanes that hold them,
-// lane j adds value j over the wavefronts (the same additions in the same
-// order) into its wavefront's totals row, and every lane reads the totals
-// back as lane-invariant loads; otherwise the results are read into scalar
-// registers.  Every wave gets the same values.
+// The totals of V values from the wavefronts' partial rows of sRed: lane j
+// adds value j over the wavefronts in wave order into `tot`, and every lane
+// then reads the totals back as lane-invariant loads.
+template <int WPS, int V>
+__device__ __forceinline__ void sum_rows(const double* sRed, double* tot, int lane, double* v) {
+    if (lane < V) {
+        double t = sRed[lane];
+#pragma unroll
+        for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + lane];
+        tot[lane] = t;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < V; ++j) v[j] = tot[j];
+}
+
+// Sums V FP64 values over the workgroup: a transposed DPP/permlane tree per
+// wavefront whose results the lanes holding them write to the wavefront's
+// row of sRed (no lane reads into scalar registers), then value j summed
+// over the wavefronts in wave order by lane j into the wavefront's totals
+// row, read back by every lane.  Every wave gets the same values; the
+// additions do not depend on scheduling.
 template <int WPS, int V>
 __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int wave) {
     static_assert(V <= kRedSlots, "reduction buffer");
+    wave_sums_to<V>(v, sRed + wave * kRedSlots, lane);
     if constexpr (WPS == 1) {
-        wave_sums_to<V>(v, sRed, lane);
         wave_lds_sync();
 #pragma unroll
         for (int j = 0; j < V; ++j) v[j] = sRed[j];
-        return;
-    } else if constexpr (WPS == 2) {
-        wave_sums_to<V>(v, sRed + wave * kRedSlots, lane);
-        __syncthreads();
-        double* tot = sRed + (2 + wave) * kRedSlots;
-        if (lane < V) tot[lane] = sRed[lane] + sRed[kRedSlots + lane];
-        wave_lds_sync();
-#pragma unroll
-        for (int j = 0; j < V; ++j) v[j] = tot[j];
-        __syncthreads();  // both have read the partials before either writes again
-        return;
-    }
-    double w[V];
-    wave_sums<V>(v, w);
-    if constexpr (WPS > 1) {
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
-        }
-        __syncthreads();
-        if constexpr (WPS >= 4) {
-            combine_lanes<WPS, V>(sRed, lane, v);
-        } else {
-#pragma unroll
-            for (int j = 0; j < V; ++j) {
-                double t = sRed[j];
-#pragma unroll
-                for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
-                v[j] = uni(t);
-            }
-        }
-        __syncthreads();
     } else {
-#pragma unroll
-        for (int j = 0; j < V; ++j) v[j] = w[j];
+        __syncthreads();
+        sum_rows<WPS, V>(sRed, sRed + (WPS + wave) * kRedSlots, lane, v);
+        __syncthreads();  // every wavefront has read the partials before any writes again
     }
 }
 
@@ -547,51 +516,14 @@ __device__ __forceinline__ void block_sum(double* v, double* sRed, int lane, int
 template <int WPS, int V>
 __device__ __forceinline__ bool block_sum_lead(double* v, double* sRed, int lane, int wave) {
     static_assert(V <= kRedSlots, "reduction buffer");
+    wave_sums_to<V>(v, sRed + wave * kRedSlots, lane);
     if constexpr (WPS == 1) {
-        wave_sums_to<V>(v, sRed, lane);
         wave_lds_sync();
 #pragma unroll
         for (int j = 0; j < V; ++j) v[j] = sRed[j];
-        return true;
-    } else if constexpr (WPS == 2) {
-        wave_sums_to<V>(v, sRed + wave * kRedSlots, lane);
-        __syncthreads();
-        if (wave == 0) {
-            double* tot = sRed + 2 * kRedSlots;
-            if (lane < V) tot[lane] = sRed[lane] + sRed[kRedSlots + lane];
-            wave_lds_sync();
-#pragma unroll
-            for (int j = 0; j < V; ++j) v[j] = tot[j];
-        }
-        return wave == 0;
-    }
-    double w[V];
-    wave_sums<V>(v, w);
-    if constexpr (WPS >= 4) {
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
-        }
-        __syncthreads();
-        if (wave == 0) combine_lanes<WPS, V>(sRed, lane, v);
-    } else if constexpr (WPS > 1) {
-        if (lane == 0 && wave > 0) {
-#pragma unroll
-            for (int j = 0; j < V; ++j) sRed[wave * kRedSlots + j] = w[j];
-        }
-        __syncthreads();
-        if (wave == 0) {
-#pragma unroll
-            for (int j = 0; j < V; ++j) {
-                double t = w[j];
-#pragma unroll
-                for (int q = 1; q < WPS; ++q) t += sRed[q * kRedSlots + j];
-                v[j] = uni(t);
-            }
-        }
     } else {
-#pragma unroll
-        for (int j = 0; j < V; ++j) v[j] = w[j];
+        __syncthreads();
+        if (wave == 0) sum_rows<WPS, V>(sRed, sRed + WPS * kRedSlots, lane, v);
     }
     return wave == 0;
 }
@@ -1159,7 +1091,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     __shared__ double sFin[1 + 2 * K];          // last iteration: ZNSSD, the projections' sums
     __shared__ double sP[12];                   // the current warp (its only copy)
     __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
-    __shared__ double sDp[M == 12 ? 12 : 1];     // order 2: the update
+    __shared__ double sDp[M];                   // the update (order 2), -H⁻¹∇ZNSSD (orders 0, 1)
     __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
     __shared__ double sHAS[M <= 6 ? 2 * M : 1];  // H⁻¹A, H⁻¹S
     __shared__ double sG[4];  // g_m, Δg, squared norm of the last update, its mean s1/n
@@ -1552,8 +1484,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 }
                 __builtin_amdgcn_wave_barrier();
                 ok = compose12_lanes(sP, sDp, lane, pn);
-#pragma unroll
-                for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
                 nrm2 = convergence_norm2_dev<M>(sDp, size);
             } else {
                 // lane j: component j of -H⁻¹∇ZNSSD, then broadcast
@@ -1568,8 +1498,12 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                     const double hs = sHAS[M + lane];
                     acc = sHAS[lane] - ratio * fma(shift - mg, hs, hb);
                 }
+                // through LDS: lane-invariant loads instead of lane reads
+                // into scalar registers
+                if (lane < M) sDp[lane] = acc;
+                wave_lds_sync();
 #pragma unroll
-                for (int k = 0; k < M; ++k) dp[k] = -readlane_f64(acc, k);
+                for (int k = 0; k < M; ++k) dp[k] = -sDp[k];
                 double p[M];
 #pragma unroll
                 for (int k = 0; k < M; ++k) p[k] = sP[k];
@@ -1602,8 +1536,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             // plan the next iteration while the other wavefronts wait
             Plan nx = pl;
             if (ok && !conv && it < A.max_it) {
-#pragma unroll
-                for (int k = 0; k < M; ++k) pn[k] = uni(pn[k]);
                 nx = make_plan<M, S>(pn, ext, xc, yc, fxc, fyc, W, A);
                 nx.diag = (kPredict && (A.hist || nrm2 < kDiagT2 * max_norm2 || it + 1 == A.max_it)) ? 1 : 0;
             }
