@@ -330,6 +330,29 @@ namespace {
 // together); the pixels of partial blocks are dealt into runs of 32 taking
 // one pixel per bank class, fullest classes first.  Sums over the subset
 // do not depend on the order beyond rounding.
+// IEEE half-precision bits of an integer, rounded to nearest even (exact
+// up to 2048 in magnitude; template offsets are far smaller).
+uint16_t half_bits(int v) {
+    if (v == 0) return 0;
+    const uint16_t sign = v < 0 ? 0x8000 : 0;
+    const uint32_t a = (uint32_t)std::abs(v);
+    int e = 31 - __builtin_clz(a);  // a in [2^e, 2^(e+1))
+    uint32_t m;
+    if (e <= 10) {
+        m = (a << (10 - e)) & 0x3ffu;
+    } else {
+        const uint32_t sh = (uint32_t)e - 10, r = a & ((1u << sh) - 1), h = 1u << (sh - 1);
+        m = (a >> sh) & 0x3ffu;
+        if (r > h || (r == h && (m & 1u))) {
+            if (++m == 0x400u) {
+                m = 0;
+                ++e;
+            }
+        }
+    }
+    return (uint16_t)(sign | (uint16_t)((e + 15) << 10) | (uint16_t)m);
+}
+
 void engine_pixel_order(const int32_t* in, int64_t n, int32_t* out) {
     auto ux = [](int32_t v) { return (int)(int16_t)(v & 0xffff); };
     auto uy = [](int32_t v) { return (int)(int16_t)((uint32_t)v >> 16); };
@@ -485,9 +508,16 @@ int dic_batch_create(dic_engine* d, int64_t n_subsets, const double* coords, con
     H2D(b->d_tpl_index, tpl_index, sizeof(int32_t) * n_subsets);
     H2D(b->d_tpl_start, tpl_start, sizeof(int64_t) * (n_templates + 1));
     H2D(b->d_tpl_xy, packed.data(), sizeof(int32_t) * n_off);
+    // the batched engine's copy: bank-friendly order, each offset pair as
+    // the two half-precision numbers its kernels keep in LDS
     std::vector<int32_t> packed_eng(n_off);
     for (int t = 0; t < n_templates; ++t) engine_pixel_order(packed.data() + tpl_start[t], tpl_start[t + 1] - tpl_start[t],
                                                             packed_eng.data() + tpl_start[t]);
+    for (int64_t k = 0; k < n_off; ++k) {
+        const int32_t o = packed_eng[k];
+        packed_eng[k] = (int32_t)((uint32_t)half_bits((int16_t)(o & 0xffff)) |
+                                  ((uint32_t)half_bits((int16_t)((uint32_t)o >> 16)) << 16));
+    }
     H2D(b->d_tpl_xy_eng, packed_eng.data(), sizeof(int32_t) * n_off);
     H2D(b->d_tpl_ext, ext.data(), sizeof(int32_t) * 4 * n_templates);
     H2D(b->d_order_perm, perm.data(), sizeof(int32_t) * n_subsets);

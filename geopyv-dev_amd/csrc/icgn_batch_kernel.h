@@ -87,7 +87,7 @@ struct BatchArgs {
     double wmax, hmax;
     int64_t n_sub;
     const SubMeta* meta;     // per subset
-    const int32_t* tpl_xy;
+    const int32_t* tpl_xy;   // offset pairs in the engine's pixel order, as OffXY (two offh_t)
     const int32_t* perm;
     const double* p0;
     double* p_out;
@@ -1153,13 +1153,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         if (tid == 0) R->status = DIC_ERR_INVALID_INPUT;
         return;
     }
-    for (int i = tid; i < n; i += NT) {
-        const int32_t o = offs[i];
-        OffXY d;
-        d.x = (offh_t)unpack_dx(o);
-        d.y = (offh_t)unpack_dy(o);
-        sXY[i] = d;
-    }
+    // the offsets arrive as the pairs the LDS keeps: a plain copy
+    for (int i = tid; i < n; i += NT) reinterpret_cast<int32_t*>(sXY)[i] = offs[i];
 
     float am[P], gxr[P], gyr[P];
 

@@ -100,7 +100,7 @@ struct Batch {
     int32_t* d_tpl_index = nullptr; // n
     int64_t* d_tpl_start = nullptr; // n_tpl + 1
     int32_t* d_tpl_xy = nullptr;    // packed (dx & 0xffff) | (dy << 16), reference scan order
-    int32_t* d_tpl_xy_eng = nullptr;  // the same pixels in LDS-bank-friendly order (batched engine)
+    int32_t* d_tpl_xy_eng = nullptr;  // the same pixels in LDS-bank-friendly order, as half-precision pairs (batched engine)
     int32_t* d_tpl_ext = nullptr;   // n_tpl x 4: dxmin, dxmax, dymin, dymax
     void* d_meta = nullptr;         // n x 48 B: per-subset start record of the batched engine
     int32_t max_side = 0;           // largest template box side
