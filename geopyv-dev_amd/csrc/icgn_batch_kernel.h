@@ -813,13 +813,35 @@ __device__ __forceinline__ void chol_inverse_column(const double* L, const doubl
 // Order-1 composition W(p)·W(dp)⁻¹ (subset.rs:499-528) with one reciprocal
 // of the determinant; false when W(dp) is singular.  No contraction, so
 // that every instantiation (whole batches and lists alike) rounds alike.
+// 1/x and 1/sqrt(x) from the hardware estimates (v_rcp_f64, v_rsq_f64)
+// refined by two Newton steps each: within an ulp or two of the correctly
+// rounded result, in a shorter dependent chain than the IEEE division and
+// square root sequences.  rsqrt_nr wants x > 0 normal (the coupled step
+// never forms y², so it does not overflow for small x).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-(x * y), y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
 __device__ __forceinline__ bool compose6_fast(const double* p, const double* dp, double* out) {
 #pragma clang fp contract(off)
     const double a = 1.0 + dp[2], b = dp[4], c = dp[0];
     const double d = dp[3], e = 1.0 + dp[5], f = dp[1];
     const double det = a * e - b * d;
     if (det == 0.0) return false;
-    const double r = 1.0 / det;
+    const double r = rcp_nr(det);
     // inverse of [[a b c] [d e f] [0 0 1]]
     const double i00 = e * r, i01 = -b * r, i02 = (b * f - c * e) * r;
     const double i10 = -d * r, i11 = a * r, i12 = (c * d - a * f) * r;
@@ -1459,8 +1481,11 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             const double s3 = fma(shift, sa, sum[2]);
             const double mg = s1 * rn;
             const double gm = f_m + mg;
-            const double dg = sqrt(fmax(s2 - s1 * mg, 0.0));
-            const double ratio = delta_f / dg;
+            const double var = fmax(s2 - s1 * mg, 0.0);
+            // 1/Δg: a zero spread keeps Δf/Δg = Δf/0 (inf, or NaN for Δf = 0)
+            const double rdg = var > 0.0 ? rsqrt_nr(var) : __builtin_inf();
+            const double dg = var > 0.0 ? var * rdg : 0.0;
+            const double ratio = delta_f * rdg;
             double pn[M];
             bool ok;
             double nrm2;

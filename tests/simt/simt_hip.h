@@ -198,6 +198,11 @@ inline simt_u32x2 __builtin_amdgcn_permlane16_swap(unsigned dst, unsigned src, b
     r.v[1] = odd ? src : d_peer;   // even VSRC0 rows take VDST's odd rows
     return r;
 }
+// v_rcp_f64 / v_rsq_f64 estimates: the exact quotient and root here (the
+// device refines its estimates by Newton steps to within an ulp or two)
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
+inline double __builtin_inf() { return INFINITY; }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline double __longlong_as_double(long long v) {
     double d;
