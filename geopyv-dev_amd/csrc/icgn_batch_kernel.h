@@ -477,6 +477,13 @@ constexpr int red_rows() {
 // then reads the totals back as lane-invariant loads.
 template <int WPS, int V>
 __device__ __forceinline__ void sum_rows(const double* sRed, double* tot, int lane, double* v) {
+    if constexpr (WPS == 2 && V <= 8) {
+        // two rows: every lane adds them itself (the same additions), one
+        // LDS round trip fewer
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = sRed[j] + sRed[kRedSlots + j];
+        return;
+    }
     if (lane < V) {
         double t = sRed[lane];
 #pragma unroll
