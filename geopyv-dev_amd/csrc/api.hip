@@ -280,6 +280,7 @@ void dic_image_destroy(dic_image* di) {
     dev_free(e, im->coef32, (size_t)im->hp * im->wp32 * sizeof(float));
     dev_free(e, im->pix_dev, im->pix_dev_bytes);
     dev_free(e, im->grey_dev, (size_t)im->h * im->w);
+    dev_free(e, im->knot, (size_t)im->h * im->w * sizeof(float4));
     delete di;
 }
 

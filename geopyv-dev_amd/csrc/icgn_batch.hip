@@ -110,6 +110,25 @@ std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm, int64
 
 namespace {
 
+// f, ∂x f and ∂y f at every pixel (knot_grad on the 5x5 block of the pixel's
+// cell, the batched engine's own setup arithmetic, so a solve reads the
+// same values it would compute); 64 x 4 pixels per workgroup
+__global__ void __launch_bounds__(256) knot_table_kernel(const float* __restrict__ C, int64_t pitch, int border, int w,
+                                                         int h, float4* __restrict__ out) {
+    const int x = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int y = (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    if (x >= w || y >= h) return;
+    const float* __restrict__ blk = C + (int64_t)(y + border - 2) * pitch + (x + border - 2);
+    float c[25];
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) c[5 * r + k] = blk[r * pitch + k];
+    float gx, gy;
+    const float v = batch::knot_grad(c, gx, gy);
+    out[(int64_t)y * w + x] = make_float4(v, gx, gy, 0.f);
+}
+
 BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
     BatchArgs A;
     A.fC = f->coef32;
@@ -127,6 +146,7 @@ BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.res = b->d_res;
     A.p0_list = nullptr;
     A.res_list = nullptr;
+    A.knot = f->knot_valid ? f->knot : nullptr;
     A.hist = prm->want_history ? b->d_hist : nullptr;
     A.hist_cap = prm->want_history ? b->hist_cap : 0;
     A.max_it = prm->max_iterations;
@@ -161,10 +181,34 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
 
 }  // namespace
 
+int knot_table_run(Engine* e, Image* im) {
+    const dim3 grid((unsigned)((im->w + 63) / 64), (unsigned)((im->h + 3) / 4));
+    knot_table_kernel<<<grid, 256, 0, e->stream>>>(im->coef32, im->wp32, im->border, (int)im->w, (int)im->h,
+                                                   im->knot);
+    DIC_HIP_TRY(hipGetLastError());
+    im->knot_valid = true;
+    return DIC_OK;
+}
+
+int knot_table_ensure(Engine* e, Image* im) {
+    if (im->knot_valid) return DIC_OK;
+    if (!im->knot) {
+        if (e->capturing) return DIC_OK;
+        const int st = dev_alloc(e, (void**)&im->knot, (size_t)im->h * im->w * sizeof(float4));
+        if (st) return st;
+    }
+    im->knot_wanted = true;
+    return knot_table_run(e, im);
+}
+
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* d_perm,
                       const int32_t* h_perm, int64_t count, bool latency, const double* p0_list,
                       dic_subset_result* res_list) {
     Engine* e = b->eng;
+    {
+        const int st = knot_table_ensure(e, const_cast<Image*>(f));
+        if (st) return st;
+    }
     BatchArgs A = common_args(b, f, g, prm);
     A.p0_list = p0_list;
     A.res_list = res_list;
@@ -203,6 +247,10 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
 // batch holds afterwards are those of the last candidate.
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm) {
     Engine* e = b->eng;
+    {
+        const int st = knot_table_ensure(e, const_cast<Image*>(f));
+        if (st) return st;
+    }
     const BatchArgs A = common_args(b, f, g, prm);
     for (int o = 0; o < 3; ++o)
         for (int w = 0; w < 9; ++w) b->tuned_wps[o][w] = 0;

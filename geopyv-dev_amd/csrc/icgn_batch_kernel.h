@@ -103,6 +103,9 @@ struct BatchArgs {
     // dispatch order (device-mapped host memory), else null
     const double* p0_list;
     dic_subset_result* res_list;
+    // f, ∂x f and ∂y f of the reference at every pixel (row pitch w), the
+    // knot form below evaluated once per image, else null
+    const float4* knot;
 };
 
 constexpr int kRedSlots = 42;  // doubles per wavefront in the cross-wave reduction buffer
@@ -1197,17 +1200,38 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         const double ay0 = yc + ext[2] - margin, ay1 = yc + ext[3] + margin;
         int c[4];
         Window W = no_window();
-        if (box_cells(ax0, ax1, ay0, ay1, A, c)) W = window_for<S>(c, A);
-        if (W.lds) stage_window<NT>(win, A.fC, A.pitch, W, tid, 0.f);
-        __syncthreads();
-        const bool fast = W.lds && ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
-        float u[V1];
-#pragma unroll
-        for (int j = 0; j < V1; ++j) u[j] = 0.f;
+        const bool inside = ax0 >= 0.0 && ax1 <= A.wmax && ay0 >= 0.0 && ay1 <= A.hmax;
         // every pixel sits at an integer offset from the centre: in the fast
         // path one basis serves the whole subset, and a centre on a whole
         // pixel puts every pixel on a knot, where the 5x5 form applies
         const bool knot = fxc == 0.f && fyc == 0.f;
+        // knots inside the image: f and ∇f come from the reference's knot
+        // table (the same operations, done once per image), read from the
+        // offsets in HBM so that the loads overlap the copy and the barrier
+        const bool table = knot && inside && A.knot != nullptr;
+        if (table) {
+            const OffXY* __restrict__ oxy = reinterpret_cast<const OffXY*>(offs);
+#pragma unroll PU
+            for (int k = 0; k < P; ++k) {
+                const int i = tid + NT * k;
+                float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (i < n) {
+                    const float2 d = off_xy(oxy[i]);
+                    t = A.knot[(int64_t)(by + (int)d.y) * A.w + (bx + (int)d.x)];
+                }
+                am[k] = t.x;
+                gxr[k] = t.y;
+                gyr[k] = t.z;
+            }
+        } else {
+            if (box_cells(ax0, ax1, ay0, ay1, A, c)) W = window_for<S>(c, A);
+            if (W.lds) stage_window<NT>(win, A.fC, A.pitch, W, tid, 0.f);
+        }
+        __syncthreads();
+        const bool fast = W.lds && inside;
+        float u[V1];
+#pragma unroll
+        for (int j = 0; j < V1; ++j) u[j] = 0.f;
         float ba[6], bb[6], bda[6], bdb[6];
         if (!knot) {
             basis_f32(fyc, ba);
@@ -1231,7 +1255,11 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             float v = 0.f, gx = 0.f, gy = 0.f;
             if (i < n) {
                 const float2 d = off_xy(sXY[i]);
-                if (fast && knot) {
+                if (table) {
+                    v = am[k];
+                    gx = gxr[k];
+                    gy = gyr[k];
+                } else if (fast && knot) {
                     float cc[25];
                     gather_block5<S>(win, cb + (int)fmaf(d.y, wwf, d.x), W.ls, cc);
                     v = knot_grad(cc, gx, gy);

@@ -72,6 +72,12 @@ struct Image {
     size_t pix_dev_bytes = 0;
     int pix_dev_type = -1;
     uint8_t* grey_dev = nullptr;  // the file loader's converted frame (h x w), loader pixel types only
+    // f, ∂x f, ∂y f at every pixel (h x w float4), made by the first
+    // batched solve that reads the image as a reference and remade by every
+    // prefilter of it from then on
+    float4* knot = nullptr;
+    bool knot_wanted = false;
+    bool knot_valid = false;
 };
 
 // Bytes per pixel of a pixel type; 0 for an unknown type.
@@ -205,6 +211,12 @@ std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm = null
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
                       const int32_t* d_perm = nullptr, const int32_t* h_perm = nullptr, int64_t count = 0,
                       bool latency = false, const double* p0_list = nullptr, dic_subset_result* res_list = nullptr);
+// the image's knot table: knot_table_ensure makes it when it is missing or
+// stale (allocating the first time, except under graph capture, where a
+// missing table leaves the solves on the window path); knot_table_run
+// recomputes it from the FP32 coefficients on the engine's stream
+int knot_table_ensure(Engine* e, Image* im);
+int knot_table_run(Engine* e, Image* im);
 // times the candidate schedules of every size class and keeps the fastest
 int autotune_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm);
 

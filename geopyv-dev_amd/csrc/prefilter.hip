@@ -465,6 +465,17 @@ __global__ void __launch_bounds__(256) loader_convert_kernel(const uint8_t* __re
 
 }  // namespace
 
+namespace {
+
+// New coefficients: the knot table of an image that solves read as a
+// reference follows them, any other image's is stale until remade.
+int knot_refresh(Engine* e, Image* img) {
+    img->knot_valid = false;
+    return img->knot_wanted ? knot_table_run(e, img) : DIC_OK;
+}
+
+}  // namespace
+
 int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem, double* tmp_own) {
     const int64_t h = img->h, w = img->w, hp = img->hp, wp = img->wp;
     if (e->prefilter_method == DIC_PREFILTER_FFT && (wp > (int64_t)kMaxOut * 1024 || hp > (int64_t)kMaxOut * 1024)) {
@@ -548,6 +559,8 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
         fir_cols_kernel<<<per_xcd * 8, 256, 0, e->stream>>>(tmp, hp, wp, k1_c, taps_c, row_tiles, n_tiles, per_xcd,
                                                             img->coef64, img->coef32, img->wp32);
         DIC_HIP_TRY(hipGetLastError());
+        const int kt = knot_refresh(e, img);
+        if (kt) return kt;
         if (!e->capturing) {
             DIC_HIP_TRY(hipEventRecord(e->ev_pre1, e->stream));
             e->pre_recorded = true;
@@ -574,6 +587,8 @@ int prefilter_run(Engine* e, Image* img, const void* pix, int pix_type, int mem,
     prefilter_cols_kernel<<<per_xcd * 8, tc, lc, e->stream>>>(tmp, hp, wp, npairs, per_xcd, sc, tw_c,
                                                               img->coef64, img->coef32, img->wp32);
     DIC_HIP_TRY(hipGetLastError());
+    const int kt = knot_refresh(e, img);
+    if (kt) return kt;
     if (!e->capturing) {
         DIC_HIP_TRY(hipEventRecord(e->ev_pre1, e->stream));
         e->pre_recorded = true;

@@ -345,7 +345,7 @@ int dic_submit(dic_server* ds, const dic_job* job, uint64_t* ticket) {
         // batch (per subset: centre, warps, result, dispatch order and start
         // record; the templates twice) and a margin
         const int64_t hp = j.h + 2 * (int64_t)j.border, wp = j.w + 2 * (int64_t)j.border;
-        q->ref_bytes = geodic::image_bytes(j.h, j.w, j.border);
+        q->ref_bytes = geodic::image_bytes(j.h, j.w, j.border) + j.h * j.w * 16;  // and its knot table
         q->bytes = q->ref_bytes + hp * wp * 8 + j.h * j.w * 8 + (int64_t)n * 512 + (int64_t)n_off * 8 + (1 << 20);
     } catch (...) {
         set_error("dic_submit: host allocation failed while copying the job");

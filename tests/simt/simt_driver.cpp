@@ -144,6 +144,24 @@ int main(int argc, char** argv) {
         A.p0_list = p0l.data();
         A.res_list = resl.data();
     }
+    // SIMT_KNOT_TABLE=1: the reference's knot table (f, ∂x f, ∂y f at every
+    // pixel by knot_grad on its 5x5 block, as the engine's table kernel
+    // makes it), so subsets centred on a whole pixel read it in setup
+    std::vector<float4> knot;
+    const char* kt = std::getenv("SIMT_KNOT_TABLE");
+    if (kt && kt[0] == '1') {
+        knot.resize((size_t)H * W);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                float c[25];
+                for (int r = 0; r < 5; ++r)
+                    for (int k = 0; k < 5; ++k) c[5 * r + k] = Cf32[(size_t)(y + B - 2 + r) * P32 + (x + B - 2 + k)];
+                float gx, gy;
+                const float val = geodic::batch::knot_grad(c, gx, gy);
+                knot[(size_t)y * W + x] = make_float4(val, gx, gy, 0.f);
+            }
+        A.knot = knot.data();
+    }
     A.hist = hist.data();
     A.hist_cap = max_it;
     A.max_it = max_it;

@@ -55,3 +55,21 @@ def test_predicted_diagnostic_equals_pass_after_loop(driver, tmp_path, order, ra
         assert r.returncode in (0, 1)  # 1: an oracle mismatch at a tiny iteration cap is not the point here
         out.append(open(path, "rb").read())
     assert len(out[0]) > 0 and out[0] == out[1]
+
+
+@pytest.mark.parametrize("order,radius,nside,wps,list_mode", [(1, 10, 3, 2, 0), (2, 10, 2, 1, 0), (1, 12, 2, 8, 1)])
+def test_knot_table_equals_window_path(driver, tmp_path, order, radius, nside, wps, list_mode):
+    # subsets centred on a whole pixel read f and its gradient from the
+    # reference's knot table in setup instead of evaluating them from the
+    # staged window; every result field must be the same bit for bit (half
+    # of the driver's subsets sit a quarter pixel off, so both setups run)
+    out = []
+    for table in ("0", "1"):
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", SIMT_KNOT_TABLE=table)
+        path = os.path.join(tmp_path, f"knot{table}.bin")
+        r = subprocess.run([driver, str(order), str(radius), str(nside), "50", "3", str(wps), str(list_mode), path],
+                           capture_output=True, text=True, timeout=600, env=env)
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        out.append(open(path, "rb").read())
+    assert len(out[0]) > 0 and out[0] == out[1]
