@@ -823,28 +823,6 @@ __device__ __forceinline__ void chol_inverse_column(const double* L, const doubl
 // Order-1 composition W(p)·W(dp)⁻¹ (subset.rs:499-528) with one reciprocal
 // of the determinant; false when W(dp) is singular.  No contraction, so
 // that every instantiation (whole batches and lists alike) rounds alike.
-// 1/x and 1/sqrt(x) from the hardware estimates (v_rcp_f64, v_rsq_f64)
-// refined by two Newton steps each: within an ulp or two of the correctly
-// rounded result, in a shorter dependent chain than the IEEE division and
-// square root sequences.  rsqrt_nr wants x > 0 normal (the coupled step
-// never forms y², so it does not overflow for small x).
-__device__ __forceinline__ double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
-__device__ __forceinline__ double rsqrt_nr(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const double e = fma(-(x * y), y, 1.0);
-        y = fma(0.5 * y, e, y);
-    }
-    return y;
-}
-
 __device__ __forceinline__ bool compose6_fast(const double* p, const double* dp, double* out) {
 #pragma clang fp contract(off)
     const double a = 1.0 + dp[2], b = dp[4], c = dp[0];
@@ -1124,6 +1102,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     __shared__ double sP[12];                   // the current warp (its only copy)
     __shared__ double sK[5];                    // f_m, Δf, Σa, 1/n, √n: read where needed
     __shared__ double sDp[M];                   // the update (order 2), -H⁻¹∇ZNSSD (orders 0, 1)
+    __shared__ double sPn[M == 12 ? 12 : 1];    // the composed warp's entries (order 2)
     __shared__ double sHv[kHinv ? M * M : 1];  // the Hessian's inverse
     __shared__ double sHAS[M <= 6 ? 2 * M : 1];  // H⁻¹A, H⁻¹S
     __shared__ double sG[4];  // g_m, Δg, squared norm of the last update, its mean s1/n
@@ -1538,7 +1517,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                     sDp[lane] = -acc;
                 }
                 __builtin_amdgcn_wave_barrier();
-                ok = compose12_lanes(sP, sDp, lane, pn);
+                ok = compose12_lanes(sP, sDp, lane, pn, sPn);
                 nrm2 = convergence_norm2_dev<M>(sDp, size);
             } else {
                 // lane j: component j of -H⁻¹∇ZNSSD, then broadcast

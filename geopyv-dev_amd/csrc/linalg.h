@@ -208,6 +208,29 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __hiloint2double(hi, lo);
 }
 
+// 1/x and 1/sqrt(x) from the hardware estimates (v_rcp_f64, v_rsq_f64)
+// refined by two Newton steps each: within an ulp or two of the correctly
+// rounded result, in a shorter dependent chain than the IEEE division and
+// square root sequences.  rsqrt_nr wants x > 0 normal (the coupled step
+// never forms y², so it does not overflow for small x).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = fma(-(x * y), y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+
 // Order-2 composition W(p)·W(dp)⁻¹ (subset.rs:547-611) spread over a
 // wavefront.  The warp is read from rows 3 and 4 of the product only
 // (subset.rs:596-609), and those rows are X·W(dp)⁻¹ with X rows 3 and 4 of
@@ -278,7 +301,8 @@ __device__ __forceinline__ double lifted_aug_entry(const double* p, const double
     return w;
 }
 
-__device__ __forceinline__ bool compose12_lanes(const double* p, const double* dp, int lane, double* out) {
+__device__ __forceinline__ bool compose12_lanes(const double* p, const double* dp, int lane, double* out,
+                                                 double* sOut) {
     const int r = lane >> 3, c = lane & 7;
     const bool live = lane < 48;
     // G[r][c] = W(dp)[c][r] for c < 6; X[0][r], X[1][r] (= W(p)[18 + r], W(p)[24 + r]) for c = 6, 7
@@ -314,24 +338,30 @@ __device__ __forceinline__ bool compose12_lanes(const double* p, const double* d
             const double t = __shfl(v, live ? src : lane, 64);
             v = live ? t : v;
         }
-        const double rp = 1.0 / readlane_f64(v, 8 * k + k);
+        const double rp = rcp_nr(readlane_f64(v, 8 * k + k));
         if (live && r == k) v *= rp;
         const double f = __shfl(v, live ? 8 * r + k : lane, 64);  // G[r][k]
         const double g = __shfl(v, live ? 8 * k + c : lane, 64);  // normalised G[k][c]
         if (live && r != k) v = fma(-f, g, v);
     }
-    out[0] = readlane_f64(v, 46);
-    out[1] = readlane_f64(v, 47);
-    out[2] = readlane_f64(v, 30) - 1.0;
-    out[3] = readlane_f64(v, 31);
-    out[4] = readlane_f64(v, 38);
-    out[5] = readlane_f64(v, 39) - 1.0;
-    out[6] = readlane_f64(v, 6) * 2.0;
-    out[7] = readlane_f64(v, 7) * 2.0;
-    out[8] = readlane_f64(v, 14);
-    out[9] = readlane_f64(v, 15);
-    out[10] = readlane_f64(v, 22) * 2.0;
-    out[11] = readlane_f64(v, 23) * 2.0;
+    // the entries through LDS (written by the lanes that hold them, read
+    // back lane-invariantly) instead of lane reads into scalar registers
+    if (live && c >= 6) sOut[2 * r + (c - 6)] = v;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    out[0] = sOut[10];
+    out[1] = sOut[11];
+    out[2] = sOut[6] - 1.0;
+    out[3] = sOut[7];
+    out[4] = sOut[8];
+    out[5] = sOut[9] - 1.0;
+    out[6] = sOut[0] * 2.0;
+    out[7] = sOut[1] * 2.0;
+    out[8] = sOut[2];
+    out[9] = sOut[3];
+    out[10] = sOut[4] * 2.0;
+    out[11] = sOut[5] * 2.0;
     return true;
 }
 
