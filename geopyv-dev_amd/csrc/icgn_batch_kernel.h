@@ -1511,10 +1511,15 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 for (int j = 0; j < M; ++j)
                     gz[j] = sAS[j] - ratio * (sum[3 + j] + shift * sAS[M + j] - mg * sAS[M + j]);
                 if (lane < M) {
-                    double acc = 0.0;
+                    // three chains of four: a shorter dependent path
+                    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
-                    for (int k = 0; k < M; ++k) acc = fma(sHv[lane * M + k], gz[k], acc);
-                    sDp[lane] = -acc;
+                    for (int k = 0; k < M / 3; ++k) {
+                        a0 = fma(sHv[lane * M + k], gz[k], a0);
+                        a1 = fma(sHv[lane * M + M / 3 + k], gz[M / 3 + k], a1);
+                        a2 = fma(sHv[lane * M + 2 * M / 3 + k], gz[2 * M / 3 + k], a2);
+                    }
+                    sDp[lane] = -((a0 + a1) + a2);
                 }
                 __builtin_amdgcn_wave_barrier();
                 ok = compose12_lanes(sP, sDp, lane, pn, sPn);
@@ -1526,9 +1531,14 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 // with ∇ZNSSD = A - ratio (B' - mg S), B' = B + shift S:
                 // H⁻¹∇ZNSSD = H⁻¹A - ratio (H⁻¹B + (shift - mg) H⁻¹S)
                 if (lane < M) {
-                    double hb = 0.0;
+                    // two chains of half the terms: a shorter dependent path
+                    double h0 = 0.0, h1 = 0.0;
 #pragma unroll
-                    for (int k = 0; k < M; ++k) hb = fma(sHv[lane * M + k], sum[3 + k], hb);
+                    for (int k = 0; k < M / 2; ++k) {
+                        h0 = fma(sHv[lane * M + k], sum[3 + k], h0);
+                        h1 = fma(sHv[lane * M + M / 2 + k], sum[3 + M / 2 + k], h1);
+                    }
+                    const double hb = h0 + h1;
                     const double hs = sHAS[M + lane];
                     acc = sHAS[lane] - ratio * fma(shift - mg, hs, hb);
                 }
@@ -1578,10 +1588,12 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
                 for (int k = 0; k < M; ++k) sP[k] = pn[k];
                 sCtl[0] = ok ? 0 : DIC_ERR_SINGULAR_WARP;
                 sCtl[1] = conv ? 1 : 0;
-                sG[0] = gm;
-                sG[1] = dg;
-                sG[2] = nrm2;
-                sG[3] = mg;
+                if (last) {
+                    sG[0] = gm;
+                    sG[1] = dg;
+                    sG[2] = nrm2;
+                    sG[3] = mg;
+                }
                 sPlan = nx;
             }
         }
