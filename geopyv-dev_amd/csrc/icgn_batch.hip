@@ -152,6 +152,8 @@ BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.max_it = prm->max_iterations;
     A.tol = prm->tolerance;
     A.max_norm = prm->max_norm;
+    A.max_norm2 = prm->max_norm * prm->max_norm;
+    A.diag_norm2 = 36.0 * A.max_norm2;  // (6 max_norm)²: the last-iteration prediction's threshold
     return A;
 }
 

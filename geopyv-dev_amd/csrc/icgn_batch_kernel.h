@@ -97,6 +97,10 @@ struct BatchArgs {
     int max_it;
     double tol;
     double max_norm;
+    // max_norm² and (6 max_norm)², set by the host: kernel arguments stay
+    // in scalar registers, where products formed in the kernel would take
+    // vector registers for the whole solve
+    double max_norm2, diag_norm2;
     int win_cap;   // floats of LDS for the coefficient window (a multiple of 4)
     int win_slack; // pixels of slack around a restaged window
     // list solves: the initial warps (12 each) and result records in
@@ -1088,7 +1092,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     // the loop, which evaluates the same warp with the same operations.
     constexpr bool kS3 = !kPredict;
     constexpr int NA0 = (kS3 ? 3 : 2) + M;  // sums of a pass not predicted to be the last
-    constexpr double kDiagT2 = 36.0;
     GEODIC_DYN_LDS(float, lds);
     float* win = lds;
     OffXY* sXY = (OffXY*)(lds + A.win_cap);
@@ -1487,7 +1490,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         const bool has_s3 = kS3 || dg_it;
         if (lead) {
             const double f_m = sK[0], delta_f = sK[1], sa = sK[2], rn = sK[3], size = sK[4];
-            const double max_norm2 = A.max_norm * A.max_norm;
+            const double max_norm2 = A.max_norm2;
             // g' above is g - (float)f_m; fold the rounding of the shift back in
             const double shift = (double)f_m32 - f_m;
             const double s1 = sum[0] + dn * shift;
@@ -1581,7 +1584,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             Plan nx = pl;
             if (ok && !conv && it < A.max_it) {
                 nx = make_plan<M, S>(pn, ext, xc, yc, fxc, fyc, W, A);
-                nx.diag = (kPredict && (A.hist || nrm2 < kDiagT2 * max_norm2 || it + 1 == A.max_it)) ? 1 : 0;
+                nx.diag = (kPredict && (A.hist || nrm2 < A.diag_norm2 || it + 1 == A.max_it)) ? 1 : 0;
             }
             if (lane == 0) {
 #pragma unroll

@@ -167,6 +167,8 @@ int main(int argc, char** argv) {
     A.max_it = max_it;
     A.tol = 0.75;
     A.max_norm = 1e-5;
+    A.max_norm2 = A.max_norm * A.max_norm;
+    A.diag_norm2 = 36.0 * A.max_norm2;
     const int wside = (ext[1] - ext[0] + 1) + 2 * slack + 8;
     const int wrow = (wside + 3 + 3) & ~3;
     const bool s72 = wrow <= 72;
