@@ -124,6 +124,20 @@ template <int M>
 __device__ __forceinline__ void warp_box(const double* p, const int32_t e[4], double& x0, double& x1, double& y0,
                                          double& y1) {
     const double cx[2] = {(double)e[0], (double)e[1]}, cy[2] = {(double)e[2], (double)e[3]};
+    if constexpr (M == 6) {
+        // the affine image of the box is bounded axis by axis: the extreme
+        // x over the corners is p0 + the extremes of (1 + p2) cx and of
+        // p4 cy (a shorter dependent chain than the four corners)
+        const double ax0 = (1.0 + p[2]) * cx[0], ax1 = (1.0 + p[2]) * cx[1];
+        const double bx0 = p[4] * cy[0], bx1 = p[4] * cy[1];
+        const double ay0 = p[3] * cx[0], ay1 = p[3] * cx[1];
+        const double by0 = (1.0 + p[5]) * cy[0], by1 = (1.0 + p[5]) * cy[1];
+        x0 = p[0] + (fmin(ax0, ax1) + fmin(bx0, bx1));
+        x1 = p[0] + (fmax(ax0, ax1) + fmax(bx0, bx1));
+        y0 = p[1] + (fmin(ay0, ay1) + fmin(by0, by1));
+        y1 = p[1] + (fmax(ay0, ay1) + fmax(by0, by1));
+        return;
+    }
     x0 = y0 = 1e300;
     x1 = y1 = -1e300;
 #pragma unroll
