@@ -1466,7 +1466,6 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     double g_m = 0.0, delta_g = 0.0, last_norm = 0.0;
     Plan pl = load_plan(&sPlan);
     for (int it = 1; it <= A.max_it; ++it) {
-        if (it > 1) pl = load_plan(&sPlan);
         const Window W = plan_window(pl);
         if (pl.restage) {
             // the target window holds C - f_m, so gathers give g' = g - f_m
@@ -1619,12 +1618,17 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         } else {
             __builtin_amdgcn_wave_barrier();
         }
-        if (sCtl[0]) {
-            status = sCtl[0];
+        // the flags and the next plan in one round of LDS reads (a last
+        // iteration left its own plan there, so the pass after the loop
+        // reads the same one)
+        const int st0 = sCtl[0], st1 = sCtl[1];
+        pl = load_plan(&sPlan);
+        if (st0) {
+            status = st0;
             break;
         }
         it_done = it;
-        if (sCtl[1]) {
+        if (st1) {
             converged = true;
             break;
         }
