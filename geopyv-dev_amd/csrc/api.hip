@@ -1083,7 +1083,8 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
         b->h_list_io = nullptr;
         b->d_list_io = nullptr;
         b->list_cap = 0;
-        DIC_HIP_TRY(hipHostMalloc((void**)&b->h_list_io, list_io_bytes(cap) + sizeof(dic_subset_result) * cap));
+        DIC_HIP_TRY(hipHostMalloc((void**)&b->h_list_io,
+                                  list_io_bytes(cap) + sizeof(dic_subset_result) * cap + list_ints_bytes(cap)));
         DIC_HIP_TRY(hipHostGetDevicePointer((void**)&b->d_list_io, b->h_list_io, 0));
         b->list_cap = cap;
     }
@@ -1097,6 +1098,10 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
     const int32_t* d_ord = reinterpret_cast<const int32_t*>(b->d_list_io + ib);
     const double* d_src = reinterpret_cast<const double*>(b->d_list_io + 2 * ib);
     dic_subset_result* d_res = reinterpret_cast<dic_subset_result*>(b->d_list_io + res_off);
+    // the batched engine's done flags, one per dispatch position
+    const size_t done_off = res_off + sizeof(dic_subset_result) * (size_t)b->list_cap;
+    int32_t* h_done = reinterpret_cast<int32_t*>(b->h_list_io + done_off);
+    int32_t* d_done = reinterpret_cast<int32_t*>(b->d_list_io + done_off);
     // the kernels read each listed subset's warp from the staging and write
     // its result record there, in their dispatch order: the list's own order
     // within each warp order for the plain kernel (one launch per order),
@@ -1137,12 +1142,31 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
             h_ord[q] = idx[pos[q]];
             std::memcpy(h_p0 + 12 * q, p0 + 12 * (int64_t)pos[q], sizeof(double) * 12);
         }
-        st = launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count, true, d_src, d_res);
+        for (int64_t q = 0; q < count; ++q) __atomic_store_n(&h_done[q], 0, __ATOMIC_RELAXED);
+        st = launch_icgn_batch(b, f, g, prm, d_ord, h_ord, count, true, d_src, d_res, d_done);
         if (st) {
             (void)hipStreamSynchronize(e->stream);  // nothing may still write the staging
             return st;
         }
-        DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+        // every workgroup raises its flag after its record, and reads the
+        // staging only before that: once all flags are up the records are
+        // complete and the staging is free, though the launch may not have
+        // retired yet (what the engine enqueues next waits for it on the
+        // stream).  Polling them sees the list end sooner than a stream
+        // synchronisation does; past a time limit, or on an error, the
+        // stream's own synchronisation decides.
+        {
+            const auto t0 = std::chrono::steady_clock::now();
+            int64_t q = 0;
+            for (uint64_t spin = 1;; ++spin) {
+                while (q < count && __atomic_load_n(&h_done[q], __ATOMIC_ACQUIRE) != 0) ++q;
+                if (q == count) break;
+                if ((spin & 4095) == 0 &&
+                    std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200))
+                    break;
+            }
+            if (q < count) DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+        }
         for (int64_t q = 0; q < count; ++q) out[pos[q]] = h_res[q];
     }
     // the batch's stored result records, history and their solve settings

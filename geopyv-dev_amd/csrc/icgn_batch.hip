@@ -146,6 +146,7 @@ BatchArgs common_args(Batch* b, const Image* f, const Image* g, const dic_solve_
     A.res = b->d_res;
     A.p0_list = nullptr;
     A.res_list = nullptr;
+    A.done_list = nullptr;
     A.knot = f->knot_valid ? f->knot : nullptr;
     A.hist = prm->want_history ? b->d_hist : nullptr;
     A.hist_cap = prm->want_history ? b->hist_cap : 0;
@@ -166,6 +167,7 @@ int launch_class(Batch* b, const BatchArgs& A, const SizeClass& c, hipStream_t s
     Ac.perm = (d_perm ? d_perm : b->d_order_perm) + c.start;
     if (A.p0_list) Ac.p0_list = A.p0_list + 12 * c.start;
     if (A.res_list) Ac.res_list = A.res_list + c.start;
+    if (A.done_list) Ac.done_list = A.done_list + c.start;
     Ac.n_sub = c.count;
     Ac.win_slack = c.slack;
     // LDS: the coefficient window (template box, slack, taps, cell rounding
@@ -205,7 +207,7 @@ int knot_table_ensure(Engine* e, Image* im) {
 
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm, const int32_t* d_perm,
                       const int32_t* h_perm, int64_t count, bool latency, const double* p0_list,
-                      dic_subset_result* res_list) {
+                      dic_subset_result* res_list, int32_t* done_list) {
     Engine* e = b->eng;
     {
         const int st = knot_table_ensure(e, const_cast<Image*>(f));
@@ -214,6 +216,7 @@ int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_
     BatchArgs A = common_args(b, f, g, prm);
     A.p0_list = p0_list;
     A.res_list = res_list;
+    A.done_list = done_list;
     std::vector<SizeClass> cls = size_classes(b, h_perm, count);
     if (latency && b->waves_per_subset == 0)
         for (SizeClass& c : cls)

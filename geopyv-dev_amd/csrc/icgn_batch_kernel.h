@@ -107,6 +107,9 @@ struct BatchArgs {
     // dispatch order (device-mapped host memory), else null
     const double* p0_list;
     dic_subset_result* res_list;
+    // list solves: a flag per dispatch position, raised (after its record)
+    // by the workgroup that solved it, else null
+    int32_t* done_list;
     // f, ∂x f and ∂y f of the reference at every pixel (row pitch w), the
     // knot form below evaluated once per image, else null
     const float4* knot;
@@ -1177,8 +1180,16 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         for (int k = 0; k < 12; ++k) R->p[k] = 0.0;
         R->status = 0;
     }
+    // list solves: the workgroup's record is complete; raised with a
+    // system-scope release so that a host that sees the flag sees the record
+    const auto list_done = [&]() {
+        if constexpr (LIST) {
+            if (tid == 0 && A.done_list) __hip_atomic_store(A.done_list + idx, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    };
     if (n == 0 || n > NT * P || !centre_ok) {
         if (tid == 0) R->status = DIC_ERR_INVALID_INPUT;
+        list_done();
         return;
     }
     // the offsets arrive as the pairs the LDS keeps: a plain copy
@@ -1442,10 +1453,12 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     }
     if (delta_f == 0.0) {
         if (tid == 0) R->status = DIC_ERR_INVALID_INPUT;
+        list_done();
         return;
     }
     if (sCtl[0]) {
         if (tid == 0) R->status = DIC_ERR_SINGULAR_HESSIAN;
+        list_done();
         return;
     }
 
@@ -1638,6 +1651,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
             R->status = status;
             R->iterations = it_done;
         }
+        list_done();
         return;
     }
     if (it_done) {
@@ -1693,6 +1707,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         R->eta_u = it_done ? sqrt(eu) : qnan;
         R->eta_v = it_done ? sqrt(ev) : qnan;
     }
+    list_done();
 }
 
 }  // namespace batch

@@ -210,7 +210,8 @@ std::vector<SizeClass> size_classes(const Batch* b, const int32_t* h_perm = null
 // p0_list / res_list: as for launch_icgn_plain, in the dispatch order
 int launch_icgn_batch(Batch* b, const Image* f, const Image* g, const dic_solve_params* prm,
                       const int32_t* d_perm = nullptr, const int32_t* h_perm = nullptr, int64_t count = 0,
-                      bool latency = false, const double* p0_list = nullptr, dic_subset_result* res_list = nullptr);
+                      bool latency = false, const double* p0_list = nullptr, dic_subset_result* res_list = nullptr,
+                      int32_t* done_list = nullptr);
 // the image's knot table: knot_table_ensure makes it when it is missing or
 // stale (allocating the first time, except under graph capture, where a
 // missing table leaves the solves on the window path); knot_table_run

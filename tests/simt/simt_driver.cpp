@@ -140,9 +140,11 @@ int main(int argc, char** argv) {
     A.res = res.data();
     std::vector<double> p0l(p0);  // dispatch order = subset order here
     std::vector<dic_subset_result> resl(n);
+    std::vector<int32_t> done(n, 0);  // list mode: each workgroup's flag, raised after its record
     if (list) {
         A.p0_list = p0l.data();
         A.res_list = resl.data();
+        A.done_list = done.data();
     }
     // SIMT_KNOT_TABLE=1: the reference's knot table (f, ∂x f, ∂y f at every
     // pixel by knot_grad on its 5x5 block, as the engine's table kernel
@@ -210,6 +212,12 @@ int main(int argc, char** argv) {
                     orders.data(), p0.data(), 0, 0.75, 1e-5, max_it, orc.data(), 1);
     if (list) res = resl;
     int bad = 0;
+    if (list)
+        for (int s = 0; s < n; ++s)
+            if (done[s] != 1) {
+                printf("s=%d: done flag %d\n", s, done[s]);
+                ++bad;
+            }
     double dumax = 0.0, dsmax = 0.0;
     for (int s = 0; s < n; ++s) {
         const double du = std::max(std::fabs(res[s].p[0] - orc[s].p[0]), std::fabs(res[s].p[1] - orc[s].p[1]));

@@ -203,6 +203,12 @@ inline simt_u32x2 __builtin_amdgcn_permlane16_swap(unsigned dst, unsigned src, b
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
 inline double __builtin_inf() { return INFINITY; }
+// system-scope release store (a list solve's done flag)
+#define __HIP_MEMORY_SCOPE_SYSTEM 0
+template <class T>
+inline void __hip_atomic_store(T* p, T v, int, int) {
+    __atomic_store_n(p, v, __ATOMIC_RELEASE);
+}
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline double __longlong_as_double(long long v) {
     double d;
