@@ -1079,7 +1079,11 @@ int solve_list(Batch* b, const Image* f, const Image* g, const dic_solve_params*
         int64_t cap = 256;
         while (cap < count) cap *= 2;
         cap = std::min<int64_t>(cap, b->n);
-        if (b->h_list_io) (void)hipHostFree(b->h_list_io);
+        // a list's launch may still be retiring after its flags went up
+        if (b->h_list_io) {
+            DIC_HIP_TRY(hipStreamSynchronize(e->stream));
+            (void)hipHostFree(b->h_list_io);
+        }
         b->h_list_io = nullptr;
         b->d_list_io = nullptr;
         b->list_cap = 0;
