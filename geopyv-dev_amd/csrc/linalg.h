@@ -338,11 +338,14 @@ __device__ __forceinline__ bool compose12_lanes(const double* p, const double* d
             const double t = __shfl(v, live ? src : lane, 64);
             v = live ? t : v;
         }
-        const double rp = rcp_nr(readlane_f64(v, 8 * k + k));
-        if (live && r == k) v *= rp;
+        // the pivot row travels unnormalised, so the reciprocal of the
+        // pivot runs while the two exchanges are in flight
+        const double dk = readlane_f64(v, 8 * k + k);
         const double f = __shfl(v, live ? 8 * r + k : lane, 64);  // G[r][k]
-        const double g = __shfl(v, live ? 8 * k + c : lane, 64);  // normalised G[k][c]
-        if (live && r != k) v = fma(-f, g, v);
+        const double g = __shfl(v, live ? 8 * k + c : lane, 64);  // G[k][c]
+        const double rp = rcp_nr(dk);
+        if (live && r == k) v *= rp;
+        if (live && r != k) v = fma(-(f * rp), g, v);
     }
     // the entries through LDS (written by the lanes that hold them, read
     // back lane-invariantly) instead of lane reads into scalar registers
