@@ -7,6 +7,9 @@
 - the four-pixel list kernel with a window row wider than 72 floats (a large
   window slack) returns exactly what the whole batch returns under the same
   schedule;
+- a reference image whose pixels change after a batched solve (which made
+  its knot table of f and its gradient) gives the next solve exactly the
+  records of an image created with the new pixels;
 - calls on one engine from two threads are serialised: reading results and
   setting warps while another thread records a frame graph leaves both
   correct.
@@ -38,7 +41,7 @@ def case():
     coords = np.array([[x, y] for y in xs for x in xs])
     p0 = np.zeros((len(coords), 12))
     p0[:, :2] = np.rint(field.warp(coords[:, 0], coords[:, 1])[:, :2])
-    return dict(eng=eng, f=f, g=g, coords=coords, p0=p0, tar=tar)
+    return dict(eng=eng, f=f, g=g, coords=coords, p0=p0, tar=tar, ref=ref)
 
 
 def _batch(c, radius=12):
@@ -143,3 +146,24 @@ def test_two_threads_on_one_engine(case):
     assert not errors, errors
     # the graph's batch solved the same frame as the eager batch
     _equal(want, b_graph.results())
+
+
+def test_reference_update_refreshes_the_knot_table(case):
+    eng = case["eng"]
+    used = _native.DeviceImage(eng, pixels=case["ref"], border=20)
+    # centres on whole pixels: every subset's setup reads the table
+    b = _batch(dict(case, coords=np.rint(case["coords"])))
+    b.solve(used, case["g"], precision=_native.PREC_COMP_F32, max_norm=1e-5)  # makes used's table
+    eng.sync()
+    # the pair reversed: the old target is the new reference
+    used.update(case["tar"])
+    fresh = _native.DeviceImage(eng, pixels=case["tar"], border=20)
+    back = _native.DeviceImage(eng, pixels=case["ref"], border=20)
+    b.set_p0(-case["p0"])
+    b.solve(used, back, precision=_native.PREC_COMP_F32, max_norm=1e-5)
+    r_used = b.results()
+    b.set_p0(-case["p0"])
+    b.solve(fresh, back, precision=_native.PREC_COMP_F32, max_norm=1e-5)
+    r_fresh = b.results()
+    assert np.mean(r_fresh["converged"]) > 0.9
+    _equal(r_used, r_fresh)
