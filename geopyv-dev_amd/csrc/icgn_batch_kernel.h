@@ -605,11 +605,13 @@ __device__ __forceinline__ void hessian_rows(const float* gxr, const float* gyr,
     constexpr int NT = 64 * WPS;
     constexpr int NE = (A1 - A0) * M - ((A1 - 1) * A1 - (A0 - 1) * A0) / 2;
     constexpr int PU = P <= 4 ? P : 1;  // a few pixels per thread: overlap them
+    // the rolled loop runs the slots the template fills (a uniform count)
+    const int KP = PU == P ? P : min(P, (n + NT - 1) / NT);
     float acc[NE > 0 ? NE : 1];
 #pragma unroll
     for (int q = 0; q < NE; ++q) acc[q] = 0.f;
 #pragma unroll PU
-    for (int k = 0; k < P; ++k) {
+    for (int k = 0; k < KP; ++k) {
         const int i = tid + NT * k;
         if (i < n) {
             const float2 d = off_xy(sXY[i]);
@@ -976,6 +978,8 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
     constexpr int PU = P <= 4 ? P : 1;
     constexpr int K = Modes<M>::K;
     static_assert(S3 || !DIAG, "a diagnostic pass sums s3");
+    // the rolled loop runs the slots the template fills (a uniform count)
+    const int KP = PU == P ? P : min(P, (n + NT - 1) / NT);
     constexpr int NA = (S3 ? 3 : 2) + M;
     constexpr int NV = DIAG ? NA + 2 * K : NA;
 #pragma unroll
@@ -1001,7 +1005,7 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
         const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
         const float wwf = (float)LS;
 #pragma unroll PU
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < KP; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
                 const float2 d = off_xy(sXY[i]);
@@ -1021,7 +1025,7 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
         }
     } else {
 #pragma unroll PU
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < KP; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
                 const float2 d = off_xy(sXY[i]);
@@ -1154,6 +1158,8 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
     const SubMeta& md = A.meta[s];
     const int64_t st = md.st;
     const int n = md.n;
+    // the rolled loops run the slots the template fills (a uniform count)
+    const int KP = PU == P ? P : min(P, (n + NT - 1) / NT);
     const int32_t* __restrict__ offs = A.tpl_xy + st;
     int32_t ext[4];
 #pragma unroll
@@ -1219,7 +1225,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         if (table) {
             const OffXY* __restrict__ oxy = reinterpret_cast<const OffXY*>(offs);
 #pragma unroll PU
-            for (int k = 0; k < P; ++k) {
+            for (int k = 0; k < KP; ++k) {
                 const int i = tid + NT * k;
                 float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (i < n) {
@@ -1257,7 +1263,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
         const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
         const float wwf = (float)LS;
 #pragma unroll PU
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < KP; ++k) {
             const int i = tid + NT * k;
             float v = 0.f, gx = 0.f, gy = 0.f;
             if (i < n) {
@@ -1387,7 +1393,7 @@ __global__ void GEODIC_BATCH_ATTRS(WPS, (M == 12 ? 1 : (P <= 4 ? 2 : GEODIC_BATC
 #pragma unroll
         for (int j = 0; j < V2; ++j) v2[j] = 0.f;
 #pragma unroll PU
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < KP; ++k) {
             const int i = tid + NT * k;
             if (i < n) {
                 const float a = am[k] - d32;
