@@ -1004,23 +1004,35 @@ __device__ __forceinline__ void pixel_pass(float* acc, const Plan& pl, const Win
         const int LS = S > 0 ? S : W.ls;
         const int cb = (by - W.cy_lo) * LS + (bx - W.cx_lo);
         const float wwf = (float)LS;
-#pragma unroll PU
-        for (int k = 0; k < KP; ++k) {
-            const int i = tid + NT * k;
-            if (i < n) {
-                const float2 d = off_xy(sXY[i]);
-                float ox, oy;
-                warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
-                const float rx = rintf(ox), ry = rintf(oy);
-                // offsets inside the window are small integers: exact in FP32
-                float cc[36];
-                gather_block<S>(win, cb + (int)fmaf(ry, wwf, rx), W.ls, cc);
-                float a[6], b[6];
-                basis_f32_s(oy - ry, a);
-                basis_f32_s(ox - rx, b);
-                const float gp = contract6r(cc, a, b);
-                accumulate<M, S3>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
-                diag(gp, d, k);
+        const auto pixel = [&](int k, int i) {
+            const float2 d = off_xy(sXY[i]);
+            float ox, oy;
+            warp_pos<M>(pl.q, c0xh, c0yh, d.x, d.y, ox, oy);
+            const float rx = rintf(ox), ry = rintf(oy);
+            // offsets inside the window are small integers: exact in FP32
+            float cc[36];
+            gather_block<S>(win, cb + (int)fmaf(ry, wwf, rx), W.ls, cc);
+            float a[6], b[6];
+            basis_f32_s(oy - ry, a);
+            basis_f32_s(ox - rx, b);
+            const float gp = contract6r(cc, a, b);
+            accumulate<M, S3>(acc, gp, am[k], gxr[k], gyr[k], d.x, d.y);
+            diag(gp, d, k);
+        };
+        if constexpr (PU == P) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const int i = tid + NT * k;
+                if (i < n) pixel(k, i);
+            }
+        } else {
+            // slots every lane fills need no per-lane test
+            const int KF = n / NT;
+#pragma unroll 1
+            for (int k = 0; k < KF; ++k) pixel(k, tid + NT * k);
+            if (KF < KP) {
+                const int i = tid + NT * KF;
+                if (i < n) pixel(KF, i);
             }
         }
     } else {
